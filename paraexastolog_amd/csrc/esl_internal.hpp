@@ -1,0 +1,143 @@
+// Internal declarations shared by the host API and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/esl.h"
+#include "model.hpp"
+
+#define ESL_HIP(call)                                                                                          \
+	do                                                                                                         \
+	{                                                                                                          \
+		hipError_t e_ = (call);                                                                                \
+		if (e_ != hipSuccess)                                                                                  \
+			throw ::esl::error(e_ == hipErrorOutOfMemory ? ESL_E_OOM : ESL_E_DEVICE,                           \
+							   std::string(#call) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +    \
+								   std::to_string(__LINE__) + ")");                                            \
+	} while (0)
+
+namespace esl {
+
+// Owning device buffer (hipMallocAsync-free on purpose: plain hipMalloc/hipFree).
+template <typename T>
+struct dbuf
+{
+	T* p = nullptr;
+	size_t n = 0;
+	dbuf() = default;
+	dbuf(const dbuf&) = delete;
+	dbuf& operator=(const dbuf&) = delete;
+	dbuf(dbuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+	dbuf& operator=(dbuf&& o) noexcept
+	{
+		reset();
+		p = o.p, n = o.n;
+		o.p = nullptr, o.n = 0;
+		return *this;
+	}
+	~dbuf() { reset(); }
+	void reset()
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr, n = 0;
+	}
+	// grows only; contents are not preserved
+	void ensure(size_t count)
+	{
+		if (count <= n && p)
+			return;
+		reset();
+		if (count == 0)
+			count = 1;
+		ESL_HIP(hipMalloc(&p, count * sizeof(T)));
+		n = count;
+	}
+	T* get() const { return p; }
+};
+
+// Device form of the rules: per node {k, input offset, table bit offset}.
+struct rule_dev_t
+{
+	int32_t k;
+	int32_t in_off;
+	int32_t tab_off; // in bits
+	int32_t pad;
+};
+
+struct graph_t;
+
+struct ctx_t
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string last_error;
+	hipEvent_t ev[16] = {};
+	// reusable scratch
+	dbuf<uint8_t> scratch_a, scratch_b;
+	dbuf<int32_t> flags;	// small device counters / flags
+	int32_t* h_flags = nullptr; // pinned mirror
+};
+
+struct solution_t
+{
+	bool valid = false;
+	std::vector<int32_t> term_indptr, term_indices;
+	std::vector<double> term_data;
+	std::vector<int32_t> nonterm_indptr, nonterm_indices;
+	std::vector<double> nonterm_data;
+};
+
+struct graph_t
+{
+	int N = 0;
+	int64_t n = 0, nnz = 0;
+	std::vector<double> rates; // rates given to esl_table_build (zero pattern)
+	uint32_t up_ok = 0, down_ok = 0;
+
+	// table
+	dbuf<rule_dev_t> rules;
+	dbuf<int32_t> rule_inputs;
+	dbuf<uint32_t> rule_tables;
+	dbuf<uint32_t> mask;   // n: bit i set iff s -> s ^ (1 << i)
+	dbuf<int32_t> indptr;  // n + 1
+	dbuf<int32_t> rows;	   // nnz (targets)
+	dbuf<int32_t> cols;	   // nnz (sources)
+
+	// graph
+	bool reorganized = false;
+	int64_t S = 0, T = 0, n_t = 0;
+	int32_t H = 0;
+	dbuf<int32_t> scc;		 // n: canonical SCC id (minimum vertex)
+	dbuf<int32_t> height;	 // n: height of the vertex's SCC
+	dbuf<int32_t> reordered; // n
+	dbuf<int32_t> offsets;	 // S + 1
+	dbuf<int32_t> rank;		 // n: position of the vertex's SCC in the canonical SCC order
+	std::vector<int32_t> h_offsets; // host copy of offsets (S + 1)
+
+	double stage_ms[2] = { 0, 0 };
+	solution_t sol;
+};
+
+// ---- kernel launchers (defined in the .hip files) ----
+void table_build(ctx_t& c, graph_t& g, const model_t& m);
+void graph_reorganize(ctx_t& c, graph_t& g);
+
+struct solve_args_t
+{
+	const double* d_rates; // [n_sets][2N]
+	const double* d_x0;	   // [n] shared
+	double* d_final;	   // [n_sets][n]
+	int n_sets;
+	esl_solve_opts opts;
+};
+void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep);
+
+// device scan helper: out[i] = sum_{j<i} in[j] (exclusive), out[n] = total; in/out int32
+void exclusive_scan_i32(ctx_t& c, const int32_t* in, int32_t* out, int64_t n);
+
+} // namespace esl

@@ -1,0 +1,475 @@
+// Graph stages on the device: SCC decomposition, condensation heights and the
+// canonical vertex order.
+//
+// Replaces transition_graph::reorganize_vertices (reference
+// cuda/src/transition_graph.cu:123-202), the GARDENIA FW-BW-trim SCC solver it
+// calls (cuda/src/sga/two_phase.cu:12-145, bitset.cu, wcc.cu) and the
+// topological_labelling / reorganize kernels (kernels/transition_graph.cu:3-52).
+//
+// The state graph is an orientation of (a subgraph of) the N-cube, so the
+// graph is never stored as an adjacency list here: the neighbours of v are
+// v ^ (1 << i), v -> v^(1<<i) iff bit i of mask[v], and v^(1<<i) -> v iff bit
+// i of mask[v ^ (1 << i)].  One wavefront owns a 6-sub-cube (64 consecutive
+// states), so the moves along bits 0..5 stay in registers (cross-lane
+// shuffles) and each pass relaxes to a wave-local fixpoint before it touches
+// HBM again; moves along bits >= 6 read the neighbouring sub-cube, which is
+// a contiguous, coalesced 256-byte block.
+//
+// SCCs: trimming, then min-colour propagation along edges and a backward
+// closure inside every colour class (the colouring SCC method).  Because a
+// colour is the minimum vertex that reaches a vertex, the SCC that survives a
+// round is named by its minimum vertex: the canonical SCC id, so the output is
+// deterministic (the reference's atomicCAS pivots, bitset.cu:415-423, are not).
+//
+// Heights: round k labels every SCC whose successors are all labelled, which
+// is the fixpoint of topological_labelling; terminal SCCs get 1.
+//
+// Order: [terminal SCCs by ascending id | transient SCCs by (height desc, id
+// desc)], vertices ascending inside an SCC (transition_graph.cu:170-198).
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "common.hip.hpp"
+#include "esl_internal.hpp"
+
+namespace esl {
+
+constexpr int gblock = 256;
+
+// ---- trimming: a vertex without an active in- or out-neighbour is a singleton SCC
+__global__ __launch_bounds__(gblock) void k_trim(int N, int64_t n, const uint32_t* __restrict__ mask,
+												 int32_t* __restrict__ scc, int32_t* __restrict__ changed)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	const bool valid = v < n;
+	const uint32_t mv = valid ? mask[v] : 0u;
+	bool active = valid && scc[v] < 0;
+	bool gin = false, gout = false;
+	if (active)
+		for (int i = 6; i < N; ++i)
+		{
+			const int64_t u = v ^ (int64_t(1) << i);
+			const bool ua = scc[u] < 0;
+			gin |= ua && ((mask[u] >> i) & 1u);
+			gout |= ua && ((mv >> i) & 1u);
+		}
+	const int lim = N < 6 ? N : 6;
+	bool trimmed = false;
+	for (;;)
+	{
+		bool in = gin, out = gout;
+		for (int i = 0; i < lim; ++i)
+		{
+			const int ua = __shfl_xor((int)active, 1 << i, wave);
+			const uint32_t mu = __shfl_xor(mv, 1 << i, wave);
+			in |= ua && ((mu >> i) & 1u);
+			out |= ua && ((mv >> i) & 1u);
+		}
+		const bool t = active && !(in && out);
+		if (!__any(t))
+			break;
+		if (t)
+		{
+			active = false;
+			trimmed = true;
+		}
+	}
+	if (trimmed)
+	{
+		scc[v] = (int32_t)v;
+		changed[0] = 1;
+	}
+}
+
+__global__ __launch_bounds__(gblock) void k_color_init(int64_t n, const int32_t* __restrict__ scc,
+													   int32_t* __restrict__ col, uint8_t* __restrict__ inscc)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (v < n)
+	{
+		col[v] = scc[v] < 0 ? (int32_t)v : INT32_MAX;
+		inscc[v] = 0;
+	}
+}
+
+// ---- min-colour propagation along edges (u -> v: col[v] = min(col[v], col[u]))
+__global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32_t* __restrict__ mask,
+												  const int32_t* __restrict__ scc, int32_t* __restrict__ col,
+												  int32_t* __restrict__ changed)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	const bool valid = v < n;
+	const bool active = valid && scc[v] < 0;
+	const uint32_t mv = valid ? mask[v] : 0u;
+	const int32_t c0 = active ? col[v] : INT32_MAX;
+	int32_t c = c0;
+	if (active)
+		for (int i = 6; i < N; ++i)
+		{
+			const int64_t u = v ^ (int64_t(1) << i);
+			if (((mask[u] >> i) & 1u) && scc[u] < 0)
+			{
+				const int32_t cu = col[u];
+				c = cu < c ? cu : c;
+			}
+		}
+	const int lim = N < 6 ? N : 6;
+	for (;;)
+	{
+		int32_t nc = c;
+		for (int i = 0; i < lim; ++i)
+		{
+			const int32_t cu = __shfl_xor(c, 1 << i, wave);
+			const uint32_t mu = __shfl_xor(mv, 1 << i, wave);
+			if (active && ((mu >> i) & 1u))
+				nc = cu < nc ? cu : nc;
+		}
+		const bool ch = nc != c;
+		c = nc;
+		if (!__any(ch))
+			break;
+	}
+	if (active && c != c0)
+	{
+		col[v] = c;
+		changed[0] = 1;
+	}
+}
+
+// ---- backward closure inside a colour class: v joins the SCC of its colour's
+// root if it has an edge into a member with the same colour.
+__global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32_t* __restrict__ mask,
+												  const int32_t* __restrict__ scc, const int32_t* __restrict__ col,
+												  uint8_t* __restrict__ inscc, int32_t* __restrict__ changed)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	const bool valid = v < n;
+	const bool active = valid && scc[v] < 0;
+	const uint32_t mv = valid ? mask[v] : 0u;
+	const int32_t c = active ? col[v] : -1;
+	const bool in0 = active && (c == (int32_t)v || inscc[v]);
+	bool in = in0;
+	if (active && !in)
+		for (int i = 6; i < N; ++i)
+		{
+			const int64_t w = v ^ (int64_t(1) << i);
+			if (((mv >> i) & 1u) && inscc[w] && col[w] == c && scc[w] < 0)
+			{
+				in = true;
+				break;
+			}
+		}
+	const int lim = N < 6 ? N : 6;
+	for (;;)
+	{
+		bool nin = in;
+		for (int i = 0; i < lim; ++i)
+		{
+			const int wi = __shfl_xor((int)in, 1 << i, wave);
+			const int32_t cw = __shfl_xor(c, 1 << i, wave);
+			if (active && ((mv >> i) & 1u) && wi && cw == c)
+				nin = true;
+		}
+		const bool ch = nin != in;
+		in = nin;
+		if (!__any(ch))
+			break;
+	}
+	if (in && !in0)
+	{
+		inscc[v] = 1;
+		changed[0] = 1;
+	}
+}
+
+__global__ __launch_bounds__(gblock) void k_assign(int64_t n, int32_t* __restrict__ scc,
+												   const int32_t* __restrict__ col, const uint8_t* __restrict__ inscc,
+												   int32_t* __restrict__ remaining)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	int left = 0;
+	if (v < n && scc[v] < 0)
+	{
+		if (inscc[v])
+			scc[v] = col[v];
+		else
+			left = 1;
+	}
+	if (__any(left) && lane_id() == 0)
+		atomicAdd(remaining, 1);
+}
+
+// ---- heights
+__global__ __launch_bounds__(gblock) void k_height_check(int N, int64_t n, const uint32_t* __restrict__ mask,
+														 const int32_t* __restrict__ scc,
+														 const int32_t* __restrict__ h, int32_t* __restrict__ blocked,
+														 int32_t cur)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (v >= n)
+		return;
+	const int32_t r = scc[v];
+	if (h[r] != 0)
+		return;
+	uint32_t rest = mask[v];
+	while (rest)
+	{
+		const int i = __ffs(rest) - 1;
+		rest &= rest - 1;
+		const int32_t rw = scc[v ^ (int64_t(1) << i)];
+		if (rw != r && h[rw] == 0)
+		{
+			blocked[r] = cur;
+			return;
+		}
+	}
+}
+
+__global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32_t* __restrict__ scc,
+														  int32_t* __restrict__ h,
+														  const int32_t* __restrict__ blocked, int32_t cur,
+														  int32_t* __restrict__ labelled)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	int got = 0;
+	if (v < n && scc[v] == (int32_t)v && h[v] == 0 && blocked[v] != cur)
+	{
+		h[v] = cur;
+		got = 1;
+	}
+	const int w = __popcll(__ballot(got));
+	if (lane_id() == 0 && w)
+		atomicAdd(labelled, w);
+}
+
+// ---- canonical order
+// root key: terminal -> (0, id); transient -> (H + 1 - h, n - 1 - id)
+__global__ __launch_bounds__(gblock) void k_root_keys(int64_t n, const int32_t* __restrict__ scc,
+													  const int32_t* __restrict__ h, int32_t H,
+													  const int32_t* __restrict__ root_pos,
+													  uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+													  int32_t* __restrict__ size)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (v >= n)
+		return;
+	const int32_t r = scc[v];
+	atomicAdd(&size[r], 1);
+	if (r != (int32_t)v)
+		return;
+	const int32_t hv = h[v];
+	const uint64_t hi = hv == 1 ? 0ull : (uint64_t)(H + 1 - hv);
+	const uint64_t lo = hv == 1 ? (uint64_t)v : (uint64_t)(n - 1 - v);
+	const int32_t p = root_pos[v];
+	keys[p] = (hi << 32) | lo;
+	vals[p] = (int32_t)v;
+}
+
+__global__ __launch_bounds__(gblock) void k_is_root(int64_t n, const int32_t* __restrict__ scc,
+													int32_t* __restrict__ flag)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (v < n)
+		flag[v] = scc[v] == (int32_t)v ? 1 : 0;
+}
+
+__global__ __launch_bounds__(gblock) void k_rank_roots(int64_t S, const int32_t* __restrict__ sorted_roots,
+													   const int32_t* __restrict__ size, int32_t* __restrict__ rank,
+													   int32_t* __restrict__ sorted_size)
+{
+	const int64_t k = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (k >= S)
+		return;
+	const int32_t r = sorted_roots[k];
+	rank[r] = (int32_t)k; // rank is indexed by root vertex here
+	sorted_size[k] = size[r];
+}
+
+__global__ __launch_bounds__(gblock) void k_vertex_keys(int64_t n, const int32_t* __restrict__ scc,
+														const int32_t* __restrict__ h,
+														const int32_t* __restrict__ root_rank,
+														uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
+														int32_t* __restrict__ vrank, int32_t* __restrict__ vheight)
+{
+	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (v >= n)
+		return;
+	const int32_t r = scc[v];
+	const int32_t k = root_rank[r];
+	keys[v] = (uint32_t)k;
+	vals[v] = (int32_t)v;
+	vrank[v] = k;
+	vheight[v] = h[r];
+}
+
+namespace {
+
+struct pass_flags
+{
+	ctx_t& c;
+	int32_t* d;
+	explicit pass_flags(ctx_t& ctx) : c(ctx), d(ctx.flags.get()) {}
+	void clear(int k) { ESL_HIP(hipMemsetAsync(d + k, 0, sizeof(int32_t), c.stream)); }
+	int32_t read(int k)
+	{
+		ESL_HIP(hipMemcpyAsync(c.h_flags + k, d + k, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		return c.h_flags[k];
+	}
+};
+
+} // namespace
+
+void graph_reorganize(ctx_t& c, graph_t& g)
+{
+	const int N = g.N;
+	const int64_t n = g.n;
+	const int nb = ceil_div(n, gblock);
+	pass_flags fl(c);
+
+	g.scc.ensure(n);
+	dbuf<int32_t> col;
+	col.ensure(n);
+	dbuf<uint8_t> inscc;
+	inscc.ensure(n);
+
+	ESL_HIP(hipEventRecord(c.ev[2], c.stream));
+	ESL_HIP(hipMemsetAsync(g.scc.get(), 0xff, n * sizeof(int32_t), c.stream));
+
+	const int max_passes = 1 << 20;
+	for (int round = 0;; ++round)
+	{
+		for (int p = 0;; ++p)
+		{
+			fl.clear(0);
+			k_trim<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), fl.d);
+			if (!fl.read(0))
+				break;
+			if (p > max_passes)
+				throw error(ESL_E_DEVICE, "trim did not converge");
+		}
+		k_color_init<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get());
+		for (int p = 0;; ++p)
+		{
+			fl.clear(0);
+			k_color<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), fl.d);
+			if (!fl.read(0))
+				break;
+			if (p > max_passes)
+				throw error(ESL_E_DEVICE, "colouring did not converge");
+		}
+		for (int p = 0;; ++p)
+		{
+			fl.clear(0);
+			k_close<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), inscc.get(), fl.d);
+			if (!fl.read(0))
+				break;
+			if (p > max_passes)
+				throw error(ESL_E_DEVICE, "closure did not converge");
+		}
+		fl.clear(1);
+		k_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get(), fl.d + 1);
+		ESL_HIP(hipGetLastError());
+		if (!fl.read(1))
+			break;
+		if (round > (1 << 24))
+			throw error(ESL_E_DEVICE, "SCC rounds did not converge");
+	}
+
+	// heights (per root), counted until every root is labelled
+	dbuf<int32_t> is_root, root_pos;
+	is_root.ensure(n);
+	root_pos.ensure(n + 1);
+	k_is_root<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), is_root.get());
+	exclusive_scan_i32(c, is_root.get(), root_pos.get(), n);
+	int32_t S = 0;
+	ESL_HIP(hipMemcpyAsync(&S, root_pos.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	g.S = S;
+
+	dbuf<int32_t> hroot, blocked;
+	hroot.ensure(n);
+	blocked.ensure(n);
+	ESL_HIP(hipMemsetAsync(hroot.get(), 0, n * sizeof(int32_t), c.stream));
+	ESL_HIP(hipMemsetAsync(blocked.get(), 0, n * sizeof(int32_t), c.stream));
+	int32_t labelled_total = 0, T = 0, cur = 0;
+	while (labelled_total < S)
+	{
+		++cur;
+		k_height_check<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), hroot.get(), blocked.get(), cur);
+		fl.clear(2);
+		k_height_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), blocked.get(), cur, fl.d + 2);
+		ESL_HIP(hipGetLastError());
+		const int32_t got = fl.read(2);
+		if (got == 0)
+			throw error(ESL_E_DEVICE, "condensation graph is not acyclic");
+		if (cur == 1)
+			T = got;
+		labelled_total += got;
+	}
+	g.T = T;
+	g.H = cur;
+
+	// sort roots by key
+	dbuf<uint64_t> rkeys, rkeys_out;
+	dbuf<int32_t> rvals, rvals_out, size;
+	rkeys.ensure(S);
+	rkeys_out.ensure(S);
+	rvals.ensure(S);
+	rvals_out.ensure(S);
+	size.ensure(n);
+	ESL_HIP(hipMemsetAsync(size.get(), 0, n * sizeof(int32_t), c.stream));
+	k_root_keys<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), g.H, root_pos.get(), rkeys.get(),
+											 rvals.get(), size.get());
+	ESL_HIP(hipGetLastError());
+	size_t tmp_bytes = 0;
+	ESL_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, rkeys.get(), rkeys_out.get(), rvals.get(),
+									  rvals_out.get(), (size_t)S, 0, 64, c.stream));
+	dbuf<uint8_t> tmp;
+	tmp.ensure(tmp_bytes);
+	ESL_HIP(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, rkeys.get(), rkeys_out.get(), rvals.get(),
+									  rvals_out.get(), (size_t)S, 0, 64, c.stream));
+	dbuf<int32_t> root_rank, sorted_size;
+	root_rank.ensure(n);
+	sorted_size.ensure(S + 1);
+	k_rank_roots<<<ceil_div(S, gblock), gblock, 0, c.stream>>>(S, rvals_out.get(), size.get(), root_rank.get(),
+																sorted_size.get());
+	g.offsets.ensure(S + 1);
+	exclusive_scan_i32(c, sorted_size.get(), g.offsets.get(), S);
+
+	// stable sort of the vertices by their SCC's rank
+	dbuf<uint32_t> vkeys, vkeys_out;
+	dbuf<int32_t> vvals;
+	vkeys.ensure(n);
+	vkeys_out.ensure(n);
+	vvals.ensure(n);
+	g.rank.ensure(n);
+	g.height.ensure(n);
+	g.reordered.ensure(n);
+	k_vertex_keys<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), root_rank.get(), vkeys.get(),
+											   vvals.get(), g.rank.get(), g.height.get());
+	ESL_HIP(hipGetLastError());
+	unsigned end_bit = 1;
+	while ((int64_t(1) << end_bit) < (int64_t)S)
+		++end_bit;
+	tmp_bytes = 0;
+	ESL_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, vkeys.get(), vkeys_out.get(), vvals.get(),
+									  g.reordered.get(), (size_t)n, 0, end_bit, c.stream));
+	tmp.ensure(tmp_bytes);
+	ESL_HIP(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, vkeys.get(), vkeys_out.get(), vvals.get(),
+									  g.reordered.get(), (size_t)n, 0, end_bit, c.stream));
+	ESL_HIP(hipEventRecord(c.ev[3], c.stream));
+	ESL_HIP(hipEventSynchronize(c.ev[3]));
+	float ms = 0;
+	ESL_HIP(hipEventElapsedTime(&ms, c.ev[2], c.ev[3]));
+	g.stage_ms[1] = ms;
+
+	g.h_offsets.resize(S + 1);
+	ESL_HIP(hipMemcpyAsync(g.h_offsets.data(), g.offsets.get(), (S + 1) * sizeof(int32_t), hipMemcpyDeviceToHost,
+						   c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	g.n_t = g.h_offsets[T];
+	g.reorganized = true;
+}
+
+} // namespace esl

@@ -1,0 +1,1213 @@
+// Numeric stages on the device: stationary vector lim exp(K t) x0.
+//
+// Replaces solver::solve (reference cuda/src/solver.cu:102-334) and
+// solve_system / splu / refactor (cuda/src/linear_system_solve.cu:249-626).
+//
+// The reference builds the kinetic matrix as CSC (scatter_rows_data,
+// kernels/solver.cu:4-45), factors it per SCC (dense cuSOLVER getrf for SCCs up
+// to 20,000 states, host sparse LU above) and solves one right-hand side per
+// terminal SCC.  On random networks at N = 20 single SCCs hold 10^5..10^6
+// states and an LU of such a block fills in by two to three orders of
+// magnitude, so this path does three things instead:
+//
+// 1. Operator without stored values.  K[s, t] = rate(t -> s) is a function of
+//    the flip mask and the 2N rates only: t = s ^ (1 << i), t -> s iff bit i of
+//    mask[t], rate = rates[2i + (s > t ? 0 : 1)] (kernels/solver.cu:33-36).
+//    Every product reads the 4-byte mask and the vector, never a value array.
+// 2. Exact pruning (x* is unchanged):
+//    - states not reachable from supp(x0) end with probability 0;
+//    - a transient state whose successors all lead to one terminal SCC sends
+//      all its mass there; only "ambiguous" transient states (reaching two or
+//      more terminal SCCs) enter a linear system.  Predecessors of an
+//      ambiguous state are ambiguous, so that system is closed:
+//          (D - A_AA) v = x0_A          (v = expected time spent per state),
+//      and terminal SCC j receives y_j = x0 mass that starts in or is bound
+//      for j plus the rate-weighted flow v_t * rate(t -> s) of every edge
+//      leaving A towards j.  This is R L x0 of solver.cu:302-309 through the
+//      single right-hand side N w = x0_nt (w = -v).
+//    - a terminal SCC of one state has pi = 1.
+// 3. Terminal kernels: for every reachable terminal SCC C of two or more
+//    states, K_C pi = 0 with the row of an anchor state a replaced by e_a
+//    (pi_a = 1), normalised afterwards.  The anchor is the state of largest
+//    mass after a few damped Jacobi sweeps of the singular system, chosen per
+//    rate set.
+//
+// Both systems are solved together, for every rate set of a batch, by
+// restarted GMRES with Jacobi (diagonal) right preconditioning: the
+// preconditioned operator is  z -> z - P^T z  with P the jump-chain matrix
+// (P[t, s] = rate(t -> s) / D_t).  Precision 0 runs GMRES in FP64; precision 1
+// runs FP32 GMRES inner solves on the FP64 residual (iterative refinement)
+// until the FP64 residual meets the tolerance.
+//
+// Vectors are set-major ([set][unknown]); the shared structure (mask, maps,
+// unknown list) is a few tens of MB and stays in the 256 MB Infinity Cache
+// while the sets stream through.
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hip.hpp"
+#include "esl_internal.hpp"
+
+namespace esl {
+
+constexpr int sblock = 256;
+constexpr int max_rhs_terms = 64; // reachable terminal SCCs tracked by bit masks
+
+// ============================================================================
+// setup: reachability, terminal masks, unknown set
+
+__global__ __launch_bounds__(sblock) void k_reach_init(int64_t n, const double* __restrict__ x0,
+													   uint8_t* __restrict__ reach)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (v < n)
+		reach[v] = x0[v] > 0.0 ? 1 : 0;
+}
+
+// v is reached if an in-neighbour is (wave-local fixpoint over bits 0..5).
+__global__ __launch_bounds__(sblock) void k_reach(int N, int64_t n, const uint32_t* __restrict__ mask,
+												  uint8_t* __restrict__ reach, int32_t* __restrict__ changed)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const bool valid = v < n;
+	const uint32_t mv = valid ? mask[v] : 0u;
+	const bool r0 = valid && reach[v];
+	bool r = r0;
+	if (valid && !r)
+		for (int i = 6; i < N; ++i)
+		{
+			const int64_t u = v ^ (int64_t(1) << i);
+			if (((mask[u] >> i) & 1u) && reach[u])
+			{
+				r = true;
+				break;
+			}
+		}
+	const int lim = N < 6 ? N : 6;
+	for (;;)
+	{
+		bool nr = r;
+		for (int i = 0; i < lim; ++i)
+		{
+			const int ru = __shfl_xor((int)r, 1 << i, wave);
+			const uint32_t mu = __shfl_xor(mv, 1 << i, wave);
+			if (valid && ru && ((mu >> i) & 1u))
+				nr = true;
+		}
+		const bool ch = nr != r;
+		r = nr;
+		if (!__any(ch))
+			break;
+	}
+	if (r && !r0)
+	{
+		reach[v] = 1;
+		changed[0] = 1;
+	}
+}
+
+// per terminal SCC k (canonical rank < T): reached if any of its states is
+__global__ __launch_bounds__(sblock) void k_term_reached(int64_t n_t, const int32_t* __restrict__ reordered,
+														 const int32_t* __restrict__ rank,
+														 const uint8_t* __restrict__ reach,
+														 int32_t* __restrict__ reached)
+{
+	const int64_t p = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (p >= n_t)
+		return;
+	const int32_t s = reordered[p];
+	if (reach[s])
+		reached[rank[s]] = 1;
+}
+
+// tm[s]: bit j set iff s reaches reachable terminal SCC j (j = compact index)
+__global__ __launch_bounds__(sblock) void k_tm_init(int64_t n, const int32_t* __restrict__ rank, int64_t T,
+													const int32_t* __restrict__ jmap,
+													const uint8_t* __restrict__ reach, uint64_t* __restrict__ tm)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (v >= n)
+		return;
+	uint64_t m = 0;
+	const int32_t k = rank[v];
+	if (reach[v] && k < T)
+	{
+		const int32_t j = jmap[k];
+		if (j >= 0 && j < max_rhs_terms)
+			m = 1ull << j;
+	}
+	tm[v] = m;
+}
+
+// pull over out-neighbours: tm[v] |= tm[w] for v -> w (reachable transient v)
+__global__ __launch_bounds__(sblock) void k_tm(int N, int64_t n, const uint32_t* __restrict__ mask,
+											   const int32_t* __restrict__ rank, int64_t T,
+											   const uint8_t* __restrict__ reach, uint64_t* __restrict__ tm,
+											   int32_t* __restrict__ changed)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const bool valid = v < n;
+	const bool act = valid && reach[v] && rank[v] >= T;
+	const uint32_t mv = valid ? mask[v] : 0u;
+	const uint64_t m0 = valid ? tm[v] : 0ull;
+	uint64_t m = m0;
+	if (act)
+		for (int i = 6; i < N; ++i)
+			if ((mv >> i) & 1u)
+				m |= tm[v ^ (int64_t(1) << i)];
+	const int lim = N < 6 ? N : 6;
+	for (;;)
+	{
+		uint64_t nm = m;
+		for (int i = 0; i < lim; ++i)
+		{
+			const uint64_t mw = __shfl_xor(m, 1 << i, wave);
+			if (act && ((mv >> i) & 1u))
+				nm |= mw;
+		}
+		const bool ch = nm != m;
+		m = nm;
+		if (!__any(ch))
+			break;
+	}
+	if (act && m != m0)
+	{
+		tm[v] = m;
+		changed[0] = 1;
+	}
+}
+
+// ambiguous transient: reachable and reaching >= 2 terminal SCCs (or every
+// reachable transient state when more than 64 terminal SCCs are reached)
+__global__ __launch_bounds__(sblock) void k_amb_flag(int64_t n, const int32_t* __restrict__ rank, int64_t T,
+													 const uint8_t* __restrict__ reach,
+													 const uint64_t* __restrict__ tm, int fallback,
+													 int32_t* __restrict__ flag)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (v >= n)
+		return;
+	int f = 0;
+	if (reach[v] && rank[v] >= T)
+		f = fallback ? 1 : (__popcll(tm[v]) >= 2);
+	flag[v] = f;
+}
+
+// terminal-system unknowns first (canonical order), then ambiguous states ascending
+__global__ __launch_bounds__(sblock) void k_fill_unknowns(int64_t n, const int32_t* __restrict__ flag,
+														  const int32_t* __restrict__ fpos, int64_t base,
+														  int32_t* __restrict__ ulist, int32_t* __restrict__ ucls,
+														  int32_t* __restrict__ pos)
+{
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (v >= n || !flag[v])
+		return;
+	const int64_t u = base + fpos[v];
+	ulist[u] = (int32_t)v;
+	ucls[u] = -1;
+	pos[v] = (int32_t)u;
+}
+
+__global__ __launch_bounds__(sblock) void k_fill_term_unknowns(int64_t count, const int32_t* __restrict__ states,
+															   const int32_t* __restrict__ seg_of,
+															   int32_t* __restrict__ ulist,
+															   int32_t* __restrict__ ucls, int32_t* __restrict__ pos)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (u >= count)
+		return;
+	const int32_t s = states[u];
+	ulist[u] = s;
+	ucls[u] = seg_of[u];
+	pos[s] = (int32_t)u;
+}
+
+// x0 mass that ends in terminal SCC j without any linear solve:
+// terminal states of j and unambiguous transient states bound for j.
+__global__ __launch_bounds__(sblock) void k_y0(int64_t n, const int32_t* __restrict__ rank, int64_t T,
+											   const int32_t* __restrict__ jmap, const uint8_t* __restrict__ reach,
+											   const uint64_t* __restrict__ tm, const int32_t* __restrict__ amb,
+											   const double* __restrict__ x0, int Tr, double* __restrict__ part)
+{
+	// part: [gridDim.x][Tr]; one slot per block and target
+	extern __shared__ double s_acc[];
+	for (int j = threadIdx.x; j < Tr; j += sblock)
+		s_acc[j] = 0.0;
+	__syncthreads();
+	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (v < n && reach[v] && !amb[v] && x0[v] != 0.0)
+	{
+		const int32_t k = rank[v];
+		int j = -1;
+		if (k < T)
+			j = jmap[k];
+		else if (tm[v])
+			j = __ffsll((unsigned long long)tm[v]) - 1;
+		if (j >= 0)
+			atomicAdd(&s_acc[j], x0[v]); // LDS atomics: order within a block only
+	}
+	__syncthreads();
+	for (int j = threadIdx.x; j < Tr; j += sblock)
+		part[(int64_t)blockIdx.x * Tr + j] = s_acc[j];
+}
+
+// ============================================================================
+// per-set coefficients
+
+struct sys_t
+{
+	int N;
+	int64_t nU;		   // unknowns
+	int64_t nUt;	   // terminal-system unknowns (prefix of the unknown list)
+	int B;			   // rate sets
+	const uint32_t* mask;
+	const int32_t* ulist; // state of unknown u
+	const int32_t* ucls;  // terminal-system segment of u, -1 for ambiguous transient
+	const int32_t* pos;	  // unknown of a state, -1 outside the systems
+	const double* rates;  // [B][2N]
+	const int32_t* anchor; // [B][n_seg] unknown index of the anchor of each segment
+	const double* invd;	   // [B][nU] 1/D' (1 for anchors)
+};
+
+__device__ __forceinline__ double rate_of(const double* __restrict__ r, int i, bool up)
+{
+	return r[2 * i + (up ? 0 : 1)];
+}
+
+// D_s = total out-rate of state s (all its transitions; up flips use rates[2i], down rates[2i+1])
+__device__ __forceinline__ double out_rate_s(uint32_t s, uint32_t m, const double* __restrict__ r)
+{
+	double d = 0.0;
+	while (m)
+	{
+		const int i = __ffs(m) - 1;
+		m &= m - 1;
+		d += r[2 * i + ((s >> i) & 1u)];
+	}
+	return d;
+}
+
+// invd[b][u] = 1 / D_s (1 for the anchor of the set's segment)
+__global__ __launch_bounds__(sblock) void k_invd(sys_t S, int n_seg, double* __restrict__ invd)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const int32_t s = S.ulist[u];
+	const int32_t cls = S.ucls[u];
+	double v;
+	if (cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u)
+		v = 1.0;
+	else
+		v = 1.0 / out_rate_s((uint32_t)s, S.mask[s], S.rates + (int64_t)b * 2 * S.N);
+	invd[(int64_t)b * S.nU + u] = v;
+}
+
+// coupled in-neighbours of s inside the systems, with the coefficient
+// rate(t -> s) / D'_t applied to the preconditioned variable z_t.
+template <typename T, typename F>
+__device__ __forceinline__ void for_coupled(const sys_t& S, int b, uint32_t s, int32_t cls, F&& f)
+{
+	const double* __restrict__ r = S.rates + (int64_t)b * 2 * S.N;
+	for (int i = 0; i < S.N; ++i)
+	{
+		const uint32_t t = s ^ (1u << i);
+		if (!((S.mask[t] >> i) & 1u))
+			continue;
+		const int32_t ut = S.pos[t];
+		if (ut < 0)
+			continue;
+		const int32_t ct = S.ucls[ut];
+		if ((ct < 0) != (cls < 0))
+			continue;
+		f(ut, rate_of(r, i, ((s >> i) & 1u) != 0));
+	}
+}
+
+// W = A_p V:  w_u = v_u - sum_t rate(t -> s) invd_t v_t   (anchor rows: w = v)
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_apply(sys_t S, int n_seg, const T* __restrict__ V,
+												  T* __restrict__ W)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const T* __restrict__ vb = V + (int64_t)b * S.nU;
+	const double* __restrict__ ib = S.invd + (int64_t)b * S.nU;
+	const int32_t s = S.ulist[u];
+	const int32_t cls = S.ucls[u];
+	double acc = (double)vb[u];
+	const bool anchor = cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u;
+	if (!anchor)
+		for_coupled<T>(S, b, (uint32_t)s, cls, [&](int32_t ut, double rate) { acc -= rate * ib[ut] * (double)vb[ut]; });
+	W[(int64_t)b * S.nU + u] = (T)acc;
+}
+
+// residual in FP64: r = rhs - A_p z, and per-block partial sums of r^2 and rhs^2
+__global__ __launch_bounds__(sblock) void k_residual(sys_t S, int n_seg, const double* __restrict__ rhs,
+													 const double* __restrict__ z, double* __restrict__ r,
+													 double* __restrict__ part)
+{
+	__shared__ double s_red[2][sblock / wave + 1];
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	double rr = 0.0, bb = 0.0;
+	if (u < S.nU)
+	{
+		const double* __restrict__ zb = z + (int64_t)b * S.nU;
+		const double* __restrict__ ib = S.invd + (int64_t)b * S.nU;
+		const int32_t s = S.ulist[u];
+		const int32_t cls = S.ucls[u];
+		double acc = zb[u];
+		const bool anchor = cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u;
+		if (!anchor)
+			for_coupled<double>(S, b, (uint32_t)s, cls,
+								[&](int32_t ut, double rate) { acc -= rate * ib[ut] * zb[ut]; });
+		const double bv = rhs[(int64_t)b * S.nU + u];
+		const double res = bv - acc;
+		r[(int64_t)b * S.nU + u] = res;
+		rr = res * res;
+		bb = bv * bv;
+	}
+	rr = block_sum<sblock>(rr, s_red[0]);
+	bb = block_sum<sblock>(bb, s_red[1]);
+	if (threadIdx.x == 0)
+	{
+		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 0] = rr;
+		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 1] = bb;
+	}
+}
+
+// out[b][k] = sum over blocks of part[b][blk][k]  (one wave per (b, k))
+__global__ __launch_bounds__(wave) void k_reduce_parts(const double* __restrict__ part, int nblk, int K,
+													   double* __restrict__ out, int64_t ostride, int accumulate)
+{
+	const int k = blockIdx.x;
+	const int b = blockIdx.y;
+	double s = 0.0;
+	for (int i = threadIdx.x; i < nblk; i += wave)
+		s += part[((int64_t)b * nblk + i) * K + k];
+	s = wave_sum(s);
+	if (threadIdx.x == 0)
+		out[(int64_t)b * ostride + k] = accumulate ? out[(int64_t)b * ostride + k] + s : s;
+}
+
+// partial dots: part[b][blk][i] = sum_u W[b][u] * V_i[b][u], i = 0..K-1
+template <typename T, int KMAX>
+__global__ __launch_bounds__(sblock) void k_dots(int64_t nU, int B, int K, const T* __restrict__ W,
+												 const T* __restrict__ V, int64_t vstride,
+												 double* __restrict__ part)
+{
+	__shared__ double s_red[sblock / wave + 1];
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	double w = 0.0;
+	if (u < nU)
+		w = (double)W[(int64_t)b * nU + u];
+	for (int i = 0; i < K; ++i)
+	{
+		double d = 0.0;
+		if (u < nU)
+			d = w * (double)V[i * vstride + (int64_t)b * nU + u];
+		d = block_sum<sblock>(d, s_red);
+		if (threadIdx.x == 0)
+			part[((int64_t)b * gridDim.x + blockIdx.x) * K + i] = d;
+	}
+}
+
+// W -= sum_i c[b][i] V_i ; partial sums of |W|^2
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __restrict__ W,
+														const T* __restrict__ V, int64_t vstride,
+														const double* __restrict__ c, double* __restrict__ part)
+{
+	__shared__ double s_red[sblock / wave + 1];
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	double w2 = 0.0;
+	if (u < nU)
+	{
+		double w = (double)W[(int64_t)b * nU + u];
+		const double* cb = c + (int64_t)b * K;
+		for (int i = 0; i < K; ++i)
+			w -= cb[i] * (double)V[i * vstride + (int64_t)b * nU + u];
+		const T wt = (T)w;
+		W[(int64_t)b * nU + u] = wt;
+		w2 = (double)wt * (double)wt;
+	}
+	w2 = block_sum<sblock>(w2, s_red);
+	if (threadIdx.x == 0)
+		part[(int64_t)b * gridDim.x + blockIdx.x] = w2;
+}
+
+// V_next = W / h  (h = 0 leaves V_next = 0)
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_scale(int64_t nU, const T* __restrict__ W,
+												  const double* __restrict__ hnorm, T* __restrict__ Vn)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= nU)
+		return;
+	const double h = hnorm[b];
+	const double inv = h > 0.0 ? 1.0 / h : 0.0;
+	Vn[(int64_t)b * nU + u] = (T)((double)W[(int64_t)b * nU + u] * inv);
+}
+
+// per-set GMRES state (one thread per set)
+struct gm_state_t
+{
+	double* H;	   // [B][(m+1) * m]  column j at H[j*(m+1) + i]
+	double* cs;	   // [B][m]
+	double* sn;	   // [B][m]
+	double* g;	   // [B][m+1]
+	double* c1;	   // [B][m+1] first-pass coefficients
+	double* c2;	   // [B][m+1] second-pass coefficients
+	double* hn;	   // [B] norm of the new vector
+	double* beta;  // [B]
+	double* bnorm; // [B]
+	double* res;   // [B] current residual estimate / |rhs|
+	int32_t* live; // [B] 1 while the set iterates in this cycle
+};
+
+// start of a cycle: beta = |r|, g = (beta, 0..), live = beta > tol*|rhs|
+__global__ void k_gm_start(int B, int m, gm_state_t st, const double* __restrict__ rr_bb, double tol,
+						   double* __restrict__ vscale)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= B)
+		return;
+	const double beta = sqrt(rr_bb[(int64_t)b * 2 + 0]);
+	const double bn = sqrt(rr_bb[(int64_t)b * 2 + 1]);
+	st.beta[b] = beta;
+	st.bnorm[b] = bn;
+	for (int i = 0; i <= m; ++i)
+		st.g[(int64_t)b * (m + 1) + i] = 0.0;
+	st.g[(int64_t)b * (m + 1)] = beta;
+	const double rel = bn > 0.0 ? beta / bn : beta;
+	st.res[b] = rel;
+	const int live = rel > tol ? 1 : 0;
+	st.live[b] = live;
+	vscale[b] = live ? beta : 0.0; // V_0 = r / beta (0 when the set is done)
+}
+
+// H[:, j] = c1 + c2, H[j+1, j] = |w|, Givens update, residual estimate
+__global__ void k_gm_givens(int B, int m, int j, gm_state_t st)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= B)
+		return;
+	double* H = st.H + (int64_t)b * (m + 1) * m;
+	double* col = H + (int64_t)j * (m + 1);
+	double* cs = st.cs + (int64_t)b * m;
+	double* sn = st.sn + (int64_t)b * m;
+	double* g = st.g + (int64_t)b * (m + 1);
+	if (!st.live[b])
+	{
+		for (int i = 0; i <= m; ++i)
+			col[i] = 0.0;
+		col[j] = 1.0;
+		cs[j] = 1.0;
+		sn[j] = 0.0;
+		st.hn[b] = 0.0;
+		return;
+	}
+	for (int i = 0; i <= j; ++i)
+		col[i] = st.c1[(int64_t)b * (m + 1) + i] + st.c2[(int64_t)b * (m + 1) + i];
+	const double hn = sqrt(st.hn[b]);
+	col[j + 1] = hn;
+	st.hn[b] = hn;
+	for (int i = 0; i < j; ++i)
+	{
+		const double a = col[i], c = col[i + 1];
+		col[i] = cs[i] * a + sn[i] * c;
+		col[i + 1] = -sn[i] * a + cs[i] * c;
+	}
+	const double a = col[j], c = col[j + 1];
+	const double rnorm = hypot(a, c);
+	double cj = 1.0, sj = 0.0;
+	if (rnorm > 0.0)
+	{
+		cj = a / rnorm;
+		sj = c / rnorm;
+	}
+	cs[j] = cj;
+	sn[j] = sj;
+	col[j] = rnorm;
+	col[j + 1] = 0.0;
+	g[j + 1] = -sj * g[j];
+	g[j] = cj * g[j];
+	const double bn = st.bnorm[b];
+	const double rel = fabs(g[j + 1]) / (bn > 0.0 ? bn : 1.0);
+	st.res[b] = rel;
+	if (!(hn > 0.0) || !(rnorm > 0.0))
+		st.live[b] = 0;
+}
+
+// y = H(0:k,0:k)^-1 g(0:k)  written into c1
+__global__ void k_gm_solve_y(int B, int m, int k, gm_state_t st)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= B)
+		return;
+	const double* H = st.H + (int64_t)b * (m + 1) * m;
+	const double* g = st.g + (int64_t)b * (m + 1);
+	double* y = st.c1 + (int64_t)b * (m + 1);
+	for (int i = k - 1; i >= 0; --i)
+	{
+		double s = g[i];
+		for (int l = i + 1; l < k; ++l)
+			s -= H[(int64_t)l * (m + 1) + i] * y[l];
+		const double d = H[(int64_t)i * (m + 1) + i];
+		y[i] = d != 0.0 ? s / d : 0.0;
+	}
+	for (int i = k; i <= m; ++i)
+		y[i] = 0.0;
+}
+
+// z += sum_i y_i V_i
+template <typename T, typename Z>
+__global__ __launch_bounds__(sblock) void k_combine(int64_t nU, int k, const T* __restrict__ V, int64_t vstride,
+													const double* __restrict__ y, int m, Z* __restrict__ z)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= nU)
+		return;
+	double acc = (double)z[(int64_t)b * nU + u];
+	const double* yb = y + (int64_t)b * (m + 1);
+	for (int i = 0; i < k; ++i)
+		acc += yb[i] * (double)V[i * vstride + (int64_t)b * nU + u];
+	z[(int64_t)b * nU + u] = (Z)acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_cast_scale(int64_t nU, const double* __restrict__ r,
+													   const double* __restrict__ scale, T* __restrict__ out)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= nU)
+		return;
+	const double s = scale[b];
+	out[(int64_t)b * nU + u] = (T)(s > 0.0 ? r[(int64_t)b * nU + u] / s : 0.0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_zero(int64_t count, T* __restrict__ p)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		p[i] = (T)0;
+}
+
+// ============================================================================
+// anchors: damped Jacobi sweeps p <- p/2 + (sum_t rate(t->s) p_t / D_s)/2 on
+// every terminal segment, then the argmax of each segment per set.
+
+__global__ __launch_bounds__(sblock) void k_anchor_sweep(sys_t S, const double* __restrict__ p_in,
+														 double* __restrict__ p_out)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nUt)
+		return;
+	const double* r = S.rates + (int64_t)b * 2 * S.N;
+	const int32_t s = S.ulist[u];
+	const int32_t cls = S.ucls[u];
+	const double* pb = p_in + (int64_t)b * S.nUt;
+	double inflow = 0.0;
+	for (int i = 0; i < S.N; ++i)
+	{
+		const uint32_t t = (uint32_t)s ^ (1u << i);
+		if (!((S.mask[t] >> i) & 1u))
+			continue;
+		const int32_t ut = S.pos[t];
+		if (ut < 0 || ut >= S.nUt || S.ucls[ut] != cls)
+			continue;
+		inflow += rate_of(r, i, ((s >> i) & 1) != 0) * pb[ut];
+	}
+	const double d = out_rate_s((uint32_t)s, S.mask[s], r);
+	p_out[(int64_t)b * S.nUt + u] = 0.5 * pb[u] + 0.5 * inflow / d;
+}
+
+// one block per (segment, set): anchor = smallest unknown index of maximal p
+__global__ __launch_bounds__(sblock) void k_anchor_pick(int64_t nUt, const int32_t* __restrict__ seg_off,
+														const double* __restrict__ p, int n_seg,
+														int32_t* __restrict__ anchor)
+{
+	__shared__ double s_v[sblock];
+	__shared__ int32_t s_i[sblock];
+	const int k = blockIdx.x;
+	const int b = blockIdx.y;
+	const int32_t a = seg_off[k], e = seg_off[k + 1];
+	double best = -1.0;
+	int32_t bi = a;
+	for (int32_t u = a + threadIdx.x; u < e; u += sblock)
+	{
+		const double v = p[(int64_t)b * nUt + u];
+		if (v > best)
+		{
+			best = v;
+			bi = u;
+		}
+	}
+	s_v[threadIdx.x] = best;
+	s_i[threadIdx.x] = bi;
+	__syncthreads();
+	for (int w = sblock / 2; w > 0; w >>= 1)
+	{
+		if (threadIdx.x < w)
+		{
+			const double v2 = s_v[threadIdx.x + w];
+			const int32_t i2 = s_i[threadIdx.x + w];
+			if (v2 > s_v[threadIdx.x] || (v2 == s_v[threadIdx.x] && i2 < s_i[threadIdx.x]))
+			{
+				s_v[threadIdx.x] = v2;
+				s_i[threadIdx.x] = i2;
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		anchor[(int64_t)b * n_seg + k] = s_i[0];
+}
+
+// right-hand sides: anchors -> 1, other terminal unknowns -> 0, ambiguous -> x0
+__global__ __launch_bounds__(sblock) void k_rhs(sys_t S, int n_seg, const double* __restrict__ x0,
+												double* __restrict__ rhs)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const int32_t cls = S.ucls[u];
+	double v;
+	if (cls >= 0)
+		v = S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u ? 1.0 : 0.0;
+	else
+		v = x0[S.ulist[u]];
+	rhs[(int64_t)b * S.nU + u] = v;
+}
+
+// ============================================================================
+// assembly of the final vector
+
+// segment sums of pi (= z * invd) over each terminal segment, per set
+__global__ __launch_bounds__(sblock) void k_seg_sum(int64_t nU, const int32_t* __restrict__ seg_off,
+													const double* __restrict__ z, const double* __restrict__ invd,
+													int n_seg, double* __restrict__ sums)
+{
+	__shared__ double s_red[sblock / wave + 1];
+	const int k = blockIdx.x;
+	const int b = blockIdx.y;
+	double acc = 0.0;
+	for (int32_t u = seg_off[k] + threadIdx.x; u < seg_off[k + 1]; u += sblock)
+		acc += z[(int64_t)b * nU + u] * invd[(int64_t)b * nU + u];
+	acc = block_sum<sblock>(acc, s_red);
+	if (threadIdx.x == 0)
+		sums[(int64_t)b * n_seg + k] = acc;
+}
+
+// flow out of the ambiguous set towards each terminal target, per block
+__global__ __launch_bounds__(sblock) void k_exit_flow(sys_t S, const int32_t* __restrict__ rank, int64_t T,
+													  const int32_t* __restrict__ jmap,
+													  const uint64_t* __restrict__ tm, const double* __restrict__ z,
+													  int Tr, double* __restrict__ part)
+{
+	extern __shared__ double s_acc[];
+	for (int j = threadIdx.x; j < Tr; j += sblock)
+		s_acc[j] = 0.0;
+	__syncthreads();
+	const int64_t u = S.nUt + (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u < S.nU)
+	{
+		const double* r = S.rates + (int64_t)b * 2 * S.N;
+		const uint32_t t = (uint32_t)S.ulist[u];
+		const double vt = z[(int64_t)b * S.nU + u] * S.invd[(int64_t)b * S.nU + u];
+		uint32_t m = S.mask[t];
+		while (m)
+		{
+			const int i = __ffs(m) - 1;
+			m &= m - 1;
+			const uint32_t s = t ^ (1u << i);
+			const int32_t us = S.pos[s];
+			if (us >= (int32_t)S.nUt)
+				continue; // stays ambiguous
+			const int32_t k = rank[s];
+			int j;
+			if (k < T)
+				j = jmap[k];
+			else
+				j = __ffsll((unsigned long long)tm[s]) - 1;
+			if (j >= 0)
+				atomicAdd(&s_acc[j], vt * rate_of(r, i, ((t >> i) & 1u) == 0));
+		}
+	}
+	__syncthreads();
+	for (int j = threadIdx.x; j < Tr; j += sblock)
+		part[((int64_t)b * gridDim.x + blockIdx.x) * Tr + j] = s_acc[j];
+}
+
+// final[b][s] for every terminal state s of a reached terminal SCC
+__global__ __launch_bounds__(sblock) void k_final(int64_t n, int64_t n_t, const int32_t* __restrict__ reordered,
+												  const int32_t* __restrict__ rank, const int32_t* __restrict__ jmap,
+												  const int32_t* __restrict__ segmap, const int32_t* __restrict__ pos,
+												  const double* __restrict__ z, const double* __restrict__ invd,
+												  const double* __restrict__ segsum, int n_seg, int64_t nU,
+												  const double* __restrict__ y, int Tr, double* __restrict__ out)
+{
+	const int64_t p = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (p >= n_t)
+		return;
+	const int32_t s = reordered[p];
+	const int32_t k = rank[s];
+	const int32_t j = jmap[k];
+	if (j < 0)
+		return;
+	double pi = 1.0;
+	const int32_t sg = segmap[k];
+	if (sg >= 0)
+	{
+		const int32_t u = pos[s];
+		pi = z[(int64_t)b * nU + u] * invd[(int64_t)b * nU + u] / segsum[(int64_t)b * n_seg + sg];
+	}
+	out[(int64_t)b * n + s] = pi * y[(int64_t)b * Tr + j];
+}
+
+__global__ __launch_bounds__(sblock) void k_y_total(int B, int Tr, const double* __restrict__ y0,
+													double* __restrict__ y)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= B * Tr)
+		return;
+	y[i] += y0[i % Tr];
+}
+
+// ============================================================================
+// host driver
+
+namespace {
+
+struct flags_t
+{
+	ctx_t& c;
+	explicit flags_t(ctx_t& x) : c(x) {}
+	void clear(int k) { ESL_HIP(hipMemsetAsync(c.flags.get() + k, 0, sizeof(int32_t), c.stream)); }
+	int32_t* dev(int k) { return c.flags.get() + k; }
+	int32_t read(int k)
+	{
+		ESL_HIP(hipMemcpyAsync(c.h_flags + k, c.flags.get() + k, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		return c.h_flags[k];
+	}
+};
+
+template <typename T>
+void read_host(ctx_t& c, std::vector<T>& h, const T* d, size_t count)
+{
+	h.resize(count);
+	if (count)
+		ESL_HIP(hipMemcpyAsync(h.data(), d, count * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <typename T>
+void write_dev(ctx_t& c, T* d, const std::vector<T>& h)
+{
+	if (!h.empty())
+		ESL_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+}
+
+struct timer_t
+{
+	ctx_t& c;
+	hipEvent_t a, b;
+	explicit timer_t(ctx_t& x) : c(x)
+	{
+		ESL_HIP(hipEventCreate(&a));
+		ESL_HIP(hipEventCreate(&b));
+		ESL_HIP(hipEventRecord(a, c.stream));
+	}
+	double lap()
+	{
+		ESL_HIP(hipEventRecord(b, c.stream));
+		ESL_HIP(hipEventSynchronize(b));
+		float ms = 0;
+		ESL_HIP(hipEventElapsedTime(&ms, a, b));
+		std::swap(a, b);
+		return ms;
+	}
+	~timer_t()
+	{
+		(void)hipEventDestroy(a);
+		(void)hipEventDestroy(b);
+	}
+};
+
+
+// GMRES(m) with Jacobi right preconditioning on A_p z = rhs, for all sets at
+// once.  z (FP64) holds the initial guess and receives the solution.  The
+// Krylov basis is kept in T; every restart recomputes the residual in FP64, so
+// with T = float each cycle is one FP32 correction of an FP64 iterate
+// (iterative refinement).
+template <typename T>
+struct gmres_t
+{
+	ctx_t& c;
+	const sys_t& S;
+	int n_seg, m;
+	dbuf<T> V, W;
+	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r;
+	dbuf<int32_t> live;
+	gm_state_t st{};
+	std::vector<double> h_res;
+	int iters = 0, cycles = 0;
+	double final_res = 0.0;
+
+	gmres_t(ctx_t& ctx, const sys_t& sys, int nseg, int restart) : c(ctx), S(sys), n_seg(nseg), m(restart)
+	{
+		const int64_t nU = S.nU, B = S.B;
+		V.ensure((size_t)(m + 1) * nU * B);
+		W.ensure((size_t)nU * B);
+		r.ensure((size_t)nU * B);
+		H.ensure((size_t)B * (m + 1) * m);
+		cs.ensure((size_t)B * m);
+		sn.ensure((size_t)B * m);
+		g.ensure((size_t)B * (m + 1));
+		c1.ensure((size_t)B * (m + 1));
+		c2.ensure((size_t)B * (m + 1));
+		hn.ensure(B);
+		beta.ensure(B);
+		bnorm.ensure(B);
+		res.ensure(B);
+		vscale.ensure(B);
+		live.ensure(B);
+		const int nblk = ceil_div(nU, sblock);
+		part.ensure((size_t)B * nblk * (m + 1) + 16);
+		rr.ensure((size_t)B * 2);
+		st = gm_state_t{ H.get(), cs.get(), sn.get(), g.get(), c1.get(), c2.get(), hn.get(),
+						 beta.get(), bnorm.get(), res.get(), live.get() };
+		h_res.resize(B);
+	}
+
+	double max_res()
+	{
+		read_host(c, h_res, res.get(), (size_t)S.B);
+		double mx = 0.0;
+		for (double v : h_res)
+			mx = std::max(mx, std::isnan(v) ? 1e300 : v);
+		return mx;
+	}
+
+	void solve(const double* rhs, double* z, double tol, int max_iters)
+	{
+		const int64_t nU = S.nU;
+		const int B = S.B;
+		const int nblk = ceil_div(nU, sblock);
+		const dim3 grid(nblk, B);
+		const int64_t vstride = nU * B;
+		const int sb = ceil_div(B, 64);
+		for (;;)
+		{
+			k_residual<<<grid, sblock, 0, c.stream>>>(S, n_seg, rhs, z, r.get(), part.get());
+			k_reduce_parts<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nblk, 2, rr.get(), 2, 0);
+			k_gm_start<<<sb, 64, 0, c.stream>>>(B, m, st, rr.get(), tol, vscale.get());
+			ESL_HIP(hipGetLastError());
+			final_res = max_res();
+			if (final_res <= tol)
+				return;
+			if (iters >= max_iters)
+				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+											   std::to_string(final_res) + " after " + std::to_string(iters) +
+											   " iterations");
+			++cycles;
+			k_cast_scale<T><<<grid, sblock, 0, c.stream>>>(nU, r.get(), vscale.get(), V.get());
+			int k = 0;
+			for (int j = 0; j < m; ++j)
+			{
+				T* Vj = V.get() + (int64_t)j * vstride;
+				k_apply<T><<<grid, sblock, 0, c.stream>>>(S, n_seg, Vj, W.get());
+				// classical Gram-Schmidt, twice
+				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
+				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
+																 part.get());
+				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
+				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
+																 part.get());
+				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
+				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st);
+				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride);
+				ESL_HIP(hipGetLastError());
+				++iters;
+				k = j + 1;
+				const double est = max_res();
+				if (est <= 0.5 * tol || iters >= max_iters)
+					break;
+			}
+			k_gm_solve_y<<<sb, 64, 0, c.stream>>>(B, m, k, st);
+			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, z);
+			ESL_HIP(hipGetLastError());
+		}
+	}
+};
+
+} // namespace
+
+void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep)
+{
+	if (!g.reorganized)
+		throw error(ESL_E_STATE, "esl_graph_reorganize must run before a solve");
+	const int N = g.N;
+	const int64_t n = g.n;
+	const int B = a.n_sets;
+	const int64_t T = g.T;
+	const int64_t n_t = g.n_t;
+	const int nb = ceil_div(n, sblock);
+	flags_t fl(c);
+	timer_t tm_(c);
+
+	// ---- reachability from supp(x0)
+	dbuf<uint8_t> reach;
+	reach.ensure(n);
+	k_reach_init<<<nb, sblock, 0, c.stream>>>(n, a.d_x0, reach.get());
+	for (int p = 0;; ++p)
+	{
+		fl.clear(0);
+		k_reach<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), reach.get(), fl.dev(0));
+		if (!fl.read(0))
+			break;
+		if (p > (1 << 24))
+			throw error(ESL_E_DEVICE, "reachability did not converge");
+	}
+	rep.ms_stage[0] = tm_.lap();
+
+	// ---- reachable terminal SCCs, compact index j, segments of size >= 2
+	dbuf<int32_t> reached;
+	reached.ensure(T);
+	ESL_HIP(hipMemsetAsync(reached.get(), 0, T * sizeof(int32_t), c.stream));
+	if (n_t)
+		k_term_reached<<<ceil_div(n_t, sblock), sblock, 0, c.stream>>>(n_t, g.reordered.get(), g.rank.get(),
+																		 reach.get(), reached.get());
+	std::vector<int32_t> h_reached;
+	read_host(c, h_reached, reached.get(), (size_t)T);
+	std::vector<int32_t> jmap(T, -1), segmap(T, -1), seg_off(1, 0), seg_src;
+	int Tr = 0, n_seg = 0;
+	for (int64_t k = 0; k < T; ++k)
+	{
+		if (!h_reached[k])
+			continue;
+		jmap[k] = Tr++;
+		const int32_t sz = g.h_offsets[k + 1] - g.h_offsets[k];
+		if (sz >= 2)
+		{
+			segmap[k] = n_seg++;
+			seg_off.push_back(seg_off.back() + sz);
+			seg_src.push_back(g.h_offsets[k]);
+		}
+	}
+	if (Tr > 4096)
+		throw error(ESL_E_ARG, "more than 4096 terminal SCCs are reachable from the initial states");
+	const bool fallback = Tr > max_rhs_terms;
+	rep.terminals_reached = Tr;
+	dbuf<int32_t> d_jmap, d_segmap;
+	d_jmap.ensure(T);
+	d_segmap.ensure(T);
+	write_dev(c, d_jmap.get(), jmap);
+	write_dev(c, d_segmap.get(), segmap);
+
+	// ---- terminal masks and ambiguous set
+	dbuf<uint64_t> tmask;
+	tmask.ensure(n);
+	k_tm_init<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, d_jmap.get(), reach.get(), tmask.get());
+	if (!fallback && Tr > 1)
+		for (int p = 0;; ++p)
+		{
+			fl.clear(0);
+			k_tm<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), g.rank.get(), T, reach.get(), tmask.get(),
+											  fl.dev(0));
+			if (!fl.read(0))
+				break;
+			if (p > (1 << 24))
+				throw error(ESL_E_DEVICE, "terminal-mask propagation did not converge");
+		}
+	dbuf<int32_t> amb, fpos;
+	amb.ensure(n);
+	fpos.ensure(n + 1);
+	if (Tr > 1)
+		k_amb_flag<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, reach.get(), tmask.get(), fallback ? 1 : 0,
+												amb.get());
+	else
+		ESL_HIP(hipMemsetAsync(amb.get(), 0, n * sizeof(int32_t), c.stream));
+	exclusive_scan_i32(c, amb.get(), fpos.get(), n);
+	int32_t nA = 0;
+	ESL_HIP(hipMemcpyAsync(&nA, fpos.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	rep.ms_stage[1] = tm_.lap();
+
+	// ---- unknowns: terminal segments (canonical order) then ambiguous states
+	const int64_t nUt = seg_off.back();
+	const int64_t nU = nUt + nA;
+	rep.unknowns = nU;
+	dbuf<int32_t> pos, ulist, ucls, d_seg_off;
+	pos.ensure(n);
+	ulist.ensure(nU + 1);
+	ucls.ensure(nU + 1);
+	d_seg_off.ensure(seg_off.size());
+	ESL_HIP(hipMemsetAsync(pos.get(), 0xff, n * sizeof(int32_t), c.stream));
+	write_dev(c, d_seg_off.get(), seg_off);
+	if (nUt)
+	{
+		// states of the segments, from the canonical order
+		std::vector<int32_t> h_reord;
+		read_host(c, h_reord, g.reordered.get(), (size_t)n_t);
+		std::vector<int32_t> states(nUt), seg_of(nUt);
+		for (int k = 0; k < n_seg; ++k)
+			for (int32_t q = 0; q < seg_off[k + 1] - seg_off[k]; ++q)
+			{
+				states[seg_off[k] + q] = h_reord[seg_src[k] + q];
+				seg_of[seg_off[k] + q] = k;
+			}
+		dbuf<int32_t> d_states, d_segof;
+		d_states.ensure(nUt);
+		d_segof.ensure(nUt);
+		write_dev(c, d_states.get(), states);
+		write_dev(c, d_segof.get(), seg_of);
+		k_fill_term_unknowns<<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, d_states.get(), d_segof.get(),
+																			 ulist.get(), ucls.get(), pos.get());
+		ESL_HIP(hipStreamSynchronize(c.stream));
+	}
+	if (nA)
+		k_fill_unknowns<<<nb, sblock, 0, c.stream>>>(n, amb.get(), fpos.get(), nUt, ulist.get(), ucls.get(),
+													 pos.get());
+
+	// ---- x0 mass needing no solve
+	dbuf<double> y0part, y0;
+	y0part.ensure((size_t)nb * std::max(Tr, 1));
+	y0.ensure(std::max(Tr, 1));
+	if (Tr)
+	{
+		k_y0<<<nb, sblock, Tr * sizeof(double), c.stream>>>(n, g.rank.get(), T, d_jmap.get(), reach.get(),
+															 tmask.get(), amb.get(), a.d_x0, Tr, y0part.get());
+		k_reduce_parts<<<dim3(Tr, 1), wave, 0, c.stream>>>(y0part.get(), nb, Tr, y0.get(), Tr, 0);
+	}
+	ESL_HIP(hipGetLastError());
+
+	// ---- per-set system coefficients
+	sys_t S{};
+	S.N = N;
+	S.nU = nU;
+	S.nUt = nUt;
+	S.B = B;
+	S.mask = g.mask.get();
+	S.ulist = ulist.get();
+	S.ucls = ucls.get();
+	S.pos = pos.get();
+	S.rates = a.d_rates;
+	dbuf<int32_t> anchor;
+	anchor.ensure((size_t)B * std::max(n_seg, 1));
+	if (n_seg)
+	{
+		dbuf<double> p0, p1;
+		p0.ensure((size_t)B * nUt);
+		p1.ensure((size_t)B * nUt);
+		std::vector<double> ones((size_t)B * nUt, 1.0);
+		write_dev(c, p0.get(), ones);
+		const dim3 gt(ceil_div(nUt, sblock), B);
+		for (int it = 0; it < 32; ++it)
+		{
+			k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, p0.get(), p1.get());
+			std::swap(p0, p1);
+		}
+		k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), p0.get(), n_seg, anchor.get());
+		ESL_HIP(hipGetLastError());
+		ESL_HIP(hipStreamSynchronize(c.stream));
+	}
+	S.anchor = anchor.get();
+	dbuf<double> invd, rhs, z;
+	invd.ensure((size_t)B * std::max<int64_t>(nU, 1));
+	rhs.ensure((size_t)B * std::max<int64_t>(nU, 1));
+	z.ensure((size_t)B * std::max<int64_t>(nU, 1));
+	S.invd = invd.get();
+	rep.ms_stage[2] = tm_.lap();
+
+	double resid = 0.0;
+	int iters = 0, cycles = 0;
+	if (nU)
+	{
+		const dim3 gu(ceil_div(nU, sblock), B);
+		k_invd<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), invd.get());
+		k_rhs<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), a.d_x0, rhs.get());
+		k_zero<double><<<ceil_div((int64_t)B * nU, sblock), sblock, 0, c.stream>>>((int64_t)B * nU, z.get());
+		ESL_HIP(hipGetLastError());
+		const double tol = a.opts.tol > 0 ? a.opts.tol : 1e-13;
+		const int max_iters = a.opts.max_iters > 0 ? a.opts.max_iters : 20000;
+		const int restart = std::min(std::max(a.opts.restart > 0 ? a.opts.restart : 32, 2), 64);
+		if (a.opts.precision == 1)
+		{
+			gmres_t<float> gm(c, S, std::max(n_seg, 1), restart);
+			gm.solve(rhs.get(), z.get(), tol, max_iters);
+			resid = gm.final_res;
+			iters = gm.iters;
+			cycles = gm.cycles;
+		}
+		else
+		{
+			gmres_t<double> gm(c, S, std::max(n_seg, 1), restart);
+			gm.solve(rhs.get(), z.get(), tol, max_iters);
+			resid = gm.final_res;
+			iters = gm.iters;
+			cycles = gm.cycles;
+		}
+	}
+	rep.residual = resid;
+	rep.iterations = iters;
+	rep.refinements = cycles;
+	rep.ms_stage[3] = tm_.lap();
+
+	// ---- absorption masses y[b][j] and the final vector
+	dbuf<double> y, segsum;
+	y.ensure((size_t)B * std::max(Tr, 1));
+	segsum.ensure((size_t)B * std::max(n_seg, 1));
+	if (Tr)
+	{
+		if (nA)
+		{
+			const int nbA = ceil_div(nA, sblock);
+			dbuf<double> fpart;
+			fpart.ensure((size_t)B * nbA * Tr);
+			k_exit_flow<<<dim3(nbA, B), sblock, Tr * sizeof(double), c.stream>>>(S, g.rank.get(), T, d_jmap.get(),
+																				 tmask.get(), z.get(), Tr,
+																				 fpart.get());
+			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(fpart.get(), nbA, Tr, y.get(), Tr, 0);
+			ESL_HIP(hipStreamSynchronize(c.stream));
+		}
+		else
+			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, y.get());
+		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, y0.get(), y.get());
+	}
+	if (n_seg)
+		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, d_seg_off.get(), z.get(), invd.get(), n_seg,
+														  segsum.get());
+	k_zero<double><<<ceil_div((int64_t)B * n, sblock), sblock, 0, c.stream>>>((int64_t)B * n, a.d_final);
+	if (n_t && Tr)
+		k_final<<<dim3(ceil_div(n_t, sblock), B), sblock, 0, c.stream>>>(
+			n, n_t, g.reordered.get(), g.rank.get(), d_jmap.get(), d_segmap.get(), pos.get(), z.get(), invd.get(),
+			segsum.get(), std::max(n_seg, 1), nU, y.get(), Tr, a.d_final);
+	ESL_HIP(hipGetLastError());
+	rep.ms_stage[4] = tm_.lap();
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	double tot = 0;
+	for (int i = 0; i < 5; ++i)
+		tot += rep.ms_stage[i];
+	rep.ms_total = tot;
+}
+
+} // namespace esl

@@ -1,0 +1,160 @@
+"""HIP path through the C ABI against the oracle and the reference goldens.
+
+Table and graph outputs are compared bit-exactly; stationary vectors within
+the reference's own tolerances on its goldens (cuda/test/test.cu) and within
+1e-12 L1 of the FP64 oracle elsewhere."""
+import numpy as np
+import pytest
+
+from oracle import pipeline as P
+from oracle.bnet import read_bnet, read_bnet_text
+from paraexastolog_amd import (initial_state, model_builder, solver, transition_graph, transition_rates,
+                               transition_table)
+from paraexastolog_amd import synth
+
+from .helpers import REAL_MODELS, SMALL_DB, golden, l1, model_path, nonzero, product_rates
+
+pytestmark = pytest.mark.gpu
+G = golden()
+
+
+def build(ctx, model, rates):
+    t = transition_table(ctx, model, rates)
+    t.construct_table()
+    g = transition_graph(ctx, t.rows, t.cols, t.indptr)
+    g.reorganize_vertices()
+    return t, g
+
+
+def run_case(ctx, name, rates_key="rates", **opts):
+    c = G[name]
+    m = model_builder().construct_model(model_path(c["model"]))
+    r = product_rates(m, c, rates_key)
+    t, g = build(ctx, m, r.rates)
+    st = initial_state(m.nodes, c.get("fixed", []), [bool(v) for v in c.get("vals", [])], c.get("p", 1.0))
+    s = solver(ctx, t, g, r, st, **opts)
+    s.solve()
+    return c, t, g, s
+
+
+def test_trans_table_toy(ctx):
+    c = G["trans_table.toy"]
+    m = model_builder().construct_model(model_path("toy.bnet"))
+    r = transition_rates(m)
+    r.generate_uniform()
+    t = transition_table(ctx, m, r.rates)
+    t.construct_table()
+    assert t.indptr.tolist() == c["indptr"]
+    assert t.cols.tolist() == c["cols"]
+
+
+def test_trans_graph_toy(ctx):
+    c = G["trans_graph.toy"]
+    m = model_builder().construct_model(model_path("toy.bnet"))
+    r = transition_rates(m)
+    r.generate_uniform()
+    _, g = build(ctx, m, r.rates)
+    assert g.reordered_vertices.tolist() == c["reordered_vertices"]
+    assert g.sccs_offsets[: g.terminals_count + 1].tolist() == c["terminal_offsets"]
+
+
+@pytest.mark.parametrize("name", ["solver.toy", "solver.toy2", "solver.toy3", "rates.toy", "zero_rate.toy",
+                                  "symbolic_no_refactor.toy", "solver.kras", "solver.mammal", "solver.cohen",
+                                  "solver.zanudo"])
+def test_reference_goldens(ctx, name):
+    c, t, g, s = run_case(ctx, name)
+    if "reordered_vertices" in c:
+        assert g.reordered_vertices.tolist() == c["reordered_vertices"]
+        assert g.sccs_offsets[: g.terminals_count + 1].tolist() == c["terminal_offsets"]
+    nz, v = nonzero(s.final_state)
+    assert nz.tolist() == c["nonzero_indices"]
+    np.testing.assert_allclose(v, c["nonzero_data"], rtol=0, atol=max(c["tol"], 1e-13))
+    assert abs(s.final_state.sum() - 1.0) < 1e-12
+    assert s.report["residual"] <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["symbolic_with_refactor.toy", "symbolic_with_refactor.cohen"])
+def test_symbolic_with_refactor(ctx, name):
+    # the table/graph built with uniform rates, solved with the changed rates
+    # (same zero pattern), as the reference's refactor path does
+    c = G[name]
+    m = model_builder().construct_model(model_path(c["model"]))
+    r0 = product_rates(m, c, "symbolic_rates")
+    t, g = build(ctx, m, r0.rates)
+    r = product_rates(m, c, "rates")
+    st = initial_state(m.nodes, c.get("fixed", []), [bool(v) for v in c.get("vals", [])], c.get("p", 1.0))
+    s = solver(ctx, t, g, r, st)
+    s.solve()
+    nz, v = nonzero(s.final_state)
+    assert nz.tolist() == c["nonzero_indices"]
+    np.testing.assert_allclose(v, c["nonzero_data"], rtol=0, atol=c["tol"])
+
+
+def oracle_graph(om, rates):
+    n = 1 << om.n_nodes
+    ip, rows, cols = P.transition_table(om, rates)
+    return n, ip, rows, cols, P.transition_graph(n, ip, rows)
+
+
+@pytest.mark.parametrize("name", [m for m in REAL_MODELS if m not in ("zanudo_expanded.bnet",)]
+                         + ["database-small/" + f for f in SMALL_DB])
+def test_table_and_graph_bit_exact(ctx, name):
+    om = read_bnet(model_path(name))
+    rates = np.exp(np.random.default_rng(len(name)).uniform(np.log(0.1), np.log(10), 2 * om.n_nodes))
+    pm = model_builder().construct_model(model_path(name))
+    t, g = build(ctx, pm, rates)
+    n, ip, rows, cols, (reord, off, T, ids) = oracle_graph(om, rates)
+    np.testing.assert_array_equal(t.indptr, ip)
+    np.testing.assert_array_equal(t.rows, rows)
+    np.testing.assert_array_equal(t.cols, cols)
+    np.testing.assert_array_equal(g.scc_ids, ids)
+    np.testing.assert_array_equal(g.reordered_vertices, reord)
+    np.testing.assert_array_equal(g.sccs_offsets, off)
+    assert g.terminals_count == T
+
+
+def test_zero_rate_edges_removed(ctx):
+    om = read_bnet(model_path("krasmodel15vars.bnet"))
+    rates = np.ones(2 * om.n_nodes)
+    rates[[3, 8, 13]] = 0.0
+    pm = model_builder().construct_model(model_path("krasmodel15vars.bnet"))
+    t, g = build(ctx, pm, rates)
+    n, ip, rows, cols, (reord, off, T, ids) = oracle_graph(om, rates)
+    np.testing.assert_array_equal(t.indptr, ip)
+    np.testing.assert_array_equal(t.rows, rows)
+    np.testing.assert_array_equal(g.reordered_vertices, reord)
+
+
+def synth_case(name, seed, nrates=1):
+    net, rng = synth.config(name, seed)
+    rates = synth.rate_sets(rng, net.N, nrates)
+    return net, rates
+
+
+@pytest.mark.parametrize("cfg,seed", [("c0", s) for s in range(6)] + [("c2", s) for s in range(4)])
+def test_synthetic_vs_oracle(ctx, cfg, seed):
+    net, rates = synth_case(cfg, seed)
+    if cfg == "c0":
+        rates[:] = 1.0
+    text = net.bnet()
+    om = read_bnet_text(text)
+    pm = model_builder().parse_model(text)
+    t, g = build(ctx, pm, rates[0])
+    n, ip, rows, cols, graph = oracle_graph(om, rates[0])
+    np.testing.assert_array_equal(t.indptr, ip)
+    np.testing.assert_array_equal(t.rows, rows)
+    np.testing.assert_array_equal(g.reordered_vertices, graph[0])
+    np.testing.assert_array_equal(g.sccs_offsets, graph[1])
+    fixed, vals = net.fixed_spec()
+    x0 = P.initial_state(om, fixed, vals)
+    st = initial_state(pm.nodes, fixed, vals)
+    s = solver(ctx, t, g, rates[0], st)
+    s.solve()
+    if cfg == "c0":
+        ref = P.solve(n, ip, rows, rates[0], x0, graph, full=True).final_state
+        assert l1(s.final_state, ref) <= 1e-12
+    else:
+        # property checks at N=16 (oracle direct solve is too slow for every seed)
+        x = s.final_state
+        assert abs(x.sum() - 1) < 1e-12 and x.min() >= -1e-14
+        assert s.report["residual"] <= 1e-13
