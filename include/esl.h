@@ -188,6 +188,16 @@ esl_status esl_device_free(esl_ctx* ctx, void* p);
 esl_status esl_memcpy_h2d(esl_ctx* ctx, void* dst, const void* src, size_t bytes);
 esl_status esl_memcpy_d2h(esl_ctx* ctx, void* dst, const void* src, size_t bytes);
 
+/* ---- kernel profiling (HIP events on the ctx stream) -----------------------
+ * enable != 0 starts collecting per-kernel launch counts, device time and
+ * algorithmic bytes (the bytes a launch must move at minimum, as defined next
+ * to each kernel); enable == 0 stops.  Starting clears earlier figures. */
+esl_status esl_profile(esl_ctx* ctx, int enable);
+/* Figures of kernel `index` (0-based, in order of first launch); returns
+ * ESL_E_ARG past the last one.  name_out receives up to name_cap bytes. */
+esl_status esl_profile_stat(esl_ctx* ctx, int index, char* name_out, int name_cap, int64_t* launches,
+							double* total_ms, double* total_bytes);
+
 /* ---- stage timing (HIP events on the ctx stream) ---------------------------
  * Milliseconds of the device work of the last esl_table_build (stage 0) and
  * esl_graph_reorganize (stage 1) on g. */

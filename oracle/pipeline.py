@@ -344,6 +344,56 @@ def solve(n, indptr, rows, rates, x0, graph, full=True):
     return sol
 
 
+def reachable(n, indptr, rows, x0):
+    """States reachable from supp(x0) (breadth-first over the CSC table)."""
+    cols = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+    A = sp.csr_matrix((np.ones(len(rows), dtype=np.float32), (rows, cols)), shape=(n, n))
+    reach = x0 > 0
+    frontier = reach.copy()
+    while frontier.any():
+        nxt = (A @ frontier.astype(np.float32)) > 0
+        new = nxt & ~reach
+        reach |= new
+        frontier = new
+    return reach
+
+
+def solve_final(n, indptr, rows, rates, x0, graph):
+    """Final vector only, on the part reachable from supp(x0): unreachable states
+    end at 0 and unreachable terminal SCCs receive nothing, so the transient
+    system shrinks to the reachable transient states; with a single reachable
+    terminal SCC all the mass ends there and no transient solve is needed.
+    Same result as solve(..., full=False) (pinned by the tests on small models)."""
+    reordered, offsets, T, _ = graph
+    reach = reachable(n, indptr, rows, x0)
+    n_t = int(offsets[T])
+    K = kinetic(n, indptr, rows, rates)
+    xs = np.zeros(n)
+    term_j = [j for j in range(T) if reach[reordered[offsets[j]:offsets[j + 1]]].any()]
+    pis = {}
+    for j in term_j:
+        V = reordered[offsets[j]:offsets[j + 1]]
+        pis[j] = terminal_kernel(K[V][:, V]) if len(V) > 1 else np.ones(1)
+    if len(term_j) == 1:
+        y = {term_j[0]: float(x0[reach].sum())}
+    else:
+        ntv = reordered[n_t:]
+        ntv = ntv[reach[ntv]]
+        tv = reordered[:n_t]
+        scc_of_t = np.repeat(np.arange(T), np.diff(offsets[: T + 1]))
+        if len(ntv):
+            w = sla.spsolve(K[ntv][:, ntv].tocsc(), x0[ntv])
+            a_ = -(K[tv][:, ntv] @ w)
+        else:
+            a_ = np.zeros(n_t)
+        yy = np.bincount(scc_of_t, weights=x0[tv] + a_, minlength=T)
+        y = {j: yy[j] for j in term_j}
+    for j in term_j:
+        V = reordered[offsets[j]:offsets[j + 1]]
+        xs[V] = pis[j] * y[j]
+    return xs
+
+
 # ----------------------------------------------------------------------------
 # end to end
 

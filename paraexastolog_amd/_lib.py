@@ -92,6 +92,9 @@ SIGNATURES = [
     ("esl_memcpy_h2d", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_memcpy_d2h", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_graph_stage_ms", C.c_int, [_VP, _P(C.c_double)]),
+    ("esl_profile", C.c_int, [_VP, C.c_int]),
+    ("esl_profile_stat", C.c_int, [_VP, C.c_int, C.c_char_p, C.c_int, _P(C.c_int64), _P(C.c_double),
+                                   _P(C.c_double)]),
 ]
 
 _lib = None

@@ -467,6 +467,45 @@ esl_status esl_solution_export(esl_ctx* ctx, const esl_graph* g, int32_t* term_i
 	});
 }
 
+// ---------------------------------------------------------------- profiling
+
+esl_status esl_profile(esl_ctx* ctx, int enable)
+{
+	return guarded(ctx, [&] {
+		require(ctx, ESL_E_ARG, "null argument");
+		set_device(ctx);
+		ESL_HIP(hipStreamSynchronize(ctx->c.stream));
+		ctx->c.prof.flush();
+		if (enable)
+			ctx->c.prof.stats.clear();
+		ctx->c.prof.on = enable != 0;
+	});
+}
+
+esl_status esl_profile_stat(esl_ctx* ctx, int index, char* name_out, int name_cap, int64_t* launches,
+							double* total_ms, double* total_bytes)
+{
+	return guarded(ctx, [&] {
+		require(ctx, ESL_E_ARG, "null argument");
+		ESL_HIP(hipStreamSynchronize(ctx->c.stream));
+		ctx->c.prof.flush();
+		const auto& st = ctx->c.prof.stats;
+		require(index >= 0 && index < (int)st.size(), ESL_E_ARG, "no such kernel statistic");
+		const auto& e = st[index];
+		if (name_out && name_cap > 0)
+		{
+			std::strncpy(name_out, e.first.c_str(), name_cap - 1);
+			name_out[name_cap - 1] = 0;
+		}
+		if (launches)
+			*launches = e.second.launches;
+		if (total_ms)
+			*total_ms = e.second.ms;
+		if (total_bytes)
+			*total_bytes = e.second.bytes;
+	});
+}
+
 // ---------------------------------------------------------------- memory
 
 esl_status esl_device_alloc(esl_ctx* ctx, size_t bytes, void** out)

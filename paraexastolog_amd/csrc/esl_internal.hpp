@@ -71,6 +71,71 @@ struct rule_dev_t
 
 struct graph_t;
 
+// Per-kernel timing with HIP events on the context stream (off by default).
+struct kstat_t
+{
+	int64_t launches = 0;
+	double ms = 0.0;
+	double bytes = 0.0; // algorithmic bytes summed over the launches
+};
+
+struct prof_t
+{
+	bool on = false;
+	struct pending_t
+	{
+		const char* name;
+		hipEvent_t a, b;
+		double bytes;
+	};
+	std::vector<pending_t> pending;
+	std::vector<hipEvent_t> pool;
+	size_t used = 0;
+	std::vector<std::pair<std::string, kstat_t>> stats;
+
+	hipEvent_t next_event()
+	{
+		if (used == pool.size())
+		{
+			hipEvent_t e;
+			if (hipEventCreate(&e) != hipSuccess)
+				throw error(ESL_E_DEVICE, "hipEventCreate failed");
+			pool.push_back(e);
+		}
+		return pool[used++];
+	}
+	kstat_t& stat(const std::string& name)
+	{
+		for (auto& p : stats)
+			if (p.first == name)
+				return p.second;
+		stats.emplace_back(name, kstat_t{});
+		return stats.back().second;
+	}
+	// resolves the recorded event pairs (the stream must be idle)
+	void flush()
+	{
+		for (auto& p : pending)
+		{
+			float ms = 0;
+			if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+			{
+				kstat_t& k = stat(p.name);
+				k.launches++;
+				k.ms += ms;
+				k.bytes += p.bytes;
+			}
+		}
+		pending.clear();
+		used = 0;
+	}
+	~prof_t()
+	{
+		for (auto e : pool)
+			(void)hipEventDestroy(e);
+	}
+};
+
 struct ctx_t
 {
 	int device = 0;
@@ -81,7 +146,30 @@ struct ctx_t
 	dbuf<uint8_t> scratch_a, scratch_b;
 	dbuf<int32_t> flags;	// small device counters / flags
 	int32_t* h_flags = nullptr; // pinned mirror
+	prof_t prof;
 };
+
+// Launch `launch` (a lambda issuing one kernel on c.stream); with profiling on,
+// bracket it with events and book `bytes` algorithmic bytes under `name`.
+template <typename F>
+inline void timed_launch(ctx_t& c, const char* name, double bytes, F&& launch)
+{
+	if (!c.prof.on)
+	{
+		launch();
+		return;
+	}
+	hipEvent_t a = c.prof.next_event(), b = c.prof.next_event();
+	ESL_HIP(hipEventRecord(a, c.stream));
+	launch();
+	ESL_HIP(hipEventRecord(b, c.stream));
+	c.prof.pending.push_back({ name, a, b, bytes });
+	if (c.prof.pending.size() > 4096)
+	{
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		c.prof.flush();
+	}
+}
 
 struct solution_t
 {

@@ -147,8 +147,8 @@ __global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32
 	const bool active = valid && scc[v] < 0;
 	const uint32_t mv = valid ? mask[v] : 0u;
 	const int32_t c = active ? col[v] : -1;
-	const bool in0 = active && (c == (int32_t)v || inscc[v]);
-	bool in = in0;
+	const bool in0 = active && inscc[v];
+	bool in = in0 || (active && c == (int32_t)v); // the colour's root is in its own SCC
 	if (active && !in)
 		for (int i = 6; i < N; ++i)
 		{
@@ -195,8 +195,9 @@ __global__ __launch_bounds__(gblock) void k_assign(int64_t n, int32_t* __restric
 		else
 			left = 1;
 	}
-	if (__any(left) && lane_id() == 0)
-		atomicAdd(remaining, 1);
+	const int cnt = __popcll(__ballot(left));
+	if (lane_id() == 0 && cnt)
+		atomicAdd(remaining, cnt);
 }
 
 // ---- heights
@@ -337,6 +338,7 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	ESL_HIP(hipMemsetAsync(g.scc.get(), 0xff, n * sizeof(int32_t), c.stream));
 
 	const int max_passes = 1 << 20;
+	int64_t prev_left = n + 1;
 	for (int round = 0;; ++round)
 	{
 		for (int p = 0;; ++p)
@@ -370,10 +372,13 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 		fl.clear(1);
 		k_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get(), fl.d + 1);
 		ESL_HIP(hipGetLastError());
-		if (!fl.read(1))
+		const int32_t left = fl.read(1);
+		if (!left)
 			break;
-		if (round > (1 << 24))
-			throw error(ESL_E_DEVICE, "SCC rounds did not converge");
+		// every round removes at least the SCC of the smallest active vertex
+		if (left >= prev_left)
+			throw error(ESL_E_DEVICE, "SCC round made no progress");
+		prev_left = left;
 	}
 
 	// heights (per root), counted until every root is labelled

@@ -242,6 +242,8 @@ __global__ __launch_bounds__(sblock) void k_y0(int64_t n, const int32_t* __restr
 		int j = -1;
 		if (k < T)
 			j = jmap[k];
+		else if (Tr == 1)
+			j = 0; // one reachable terminal SCC: every reachable state ends there
 		else if (tm[v])
 			j = __ffsll((unsigned long long)tm[v]) - 1;
 		if (j >= 0)
@@ -789,6 +791,219 @@ __global__ __launch_bounds__(sblock) void k_y_total(int B, int Tr, const double*
 	y[i] += y0[i % Tr];
 }
 
+
+// ============================================================================
+// One workgroup per rate set: the whole restarted GMRES of that set in one
+// launch.  For systems up to a few tens of thousands of unknowns per set this
+// keeps every reduction inside the workgroup (no partial sums through HBM, no
+// host round trip per iteration); a batch of rate sets is one launch of
+// n_sets workgroups.  The Krylov basis lives in global memory (it stays in
+// L2); the Hessenberg matrix and the rotations live in LDS.
+
+constexpr int pblock = 256;
+constexpr int pwaves = pblock / wave;
+constexpr int pm_max = 32;
+
+struct pgm_out_t
+{
+	int32_t* iters;	  // [B]
+	int32_t* cycles;  // [B]
+	double* res;	  // [B] final relative FP64 residual
+};
+
+// sum over the workgroup of one value per thread, result broadcast
+__device__ __forceinline__ double pg_sum(double v, double* lds)
+{
+	v = wave_sum(v);
+	__syncthreads();
+	if (lane_id() == 0)
+		lds[threadIdx.x / wave] = v;
+	__syncthreads();
+	double t = 0.0;
+	for (int w = 0; w < pwaves; ++w)
+		t += lds[w];
+	return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double pg_apply_one(const sys_t& S, int n_seg, int b, int64_t u, const T* __restrict__ vb,
+											   const double* __restrict__ ib)
+{
+	const int32_t s = S.ulist[u];
+	const int32_t cls = S.ucls[u];
+	double acc = (double)vb[u];
+	const bool anchor = cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u;
+	if (!anchor)
+		for_coupled<T>(S, b, (uint32_t)s, cls, [&](int32_t ut, double rate) { acc -= rate * ib[ut] * (double)vb[ut]; });
+	return acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const double* __restrict__ rhs,
+													  double* __restrict__ z, T* __restrict__ Vall,
+													  double* __restrict__ rall, int m, double tol, int max_iters,
+													  pgm_out_t out)
+{
+	__shared__ double s_H[(pm_max + 1) * pm_max];
+	__shared__ double s_cs[pm_max], s_sn[pm_max], s_g[pm_max + 1], s_c[pm_max + 1];
+	__shared__ double s_part[(pm_max + 1) * pwaves];
+	__shared__ double s_red[pwaves];
+	__shared__ double s_scalar[4];
+
+	const int b = blockIdx.x;
+	const int64_t nU = S.nU;
+	const int64_t off = (int64_t)b * nU;
+	const double* __restrict__ rb = rhs + off;
+	double* __restrict__ zb = z + off;
+	double* __restrict__ r = rall + off;
+	const double* __restrict__ ib = S.invd + off;
+	T* __restrict__ V = Vall + (int64_t)b * (m + 1) * nU;
+	int iters = 0, cycles = 0;
+	double rel = 0.0;
+
+	for (;;)
+	{
+		// FP64 residual of the current iterate
+		double rr = 0.0, bb = 0.0;
+		for (int64_t u = threadIdx.x; u < nU; u += pblock)
+		{
+			const double res = rb[u] - pg_apply_one<double>(S, n_seg, b, u, zb, ib);
+			r[u] = res;
+			rr += res * res;
+			bb += rb[u] * rb[u];
+		}
+		rr = pg_sum(rr, s_red);
+		bb = pg_sum(bb, s_red);
+		const double beta = sqrt(rr);
+		const double bn = bb > 0.0 ? sqrt(bb) : 1.0;
+		rel = beta / bn;
+		if (rel <= tol || iters >= max_iters)
+			break;
+		++cycles;
+		const double inv_beta = 1.0 / beta;
+		for (int64_t u = threadIdx.x; u < nU; u += pblock)
+			V[u] = (T)(r[u] * inv_beta);
+		if (threadIdx.x == 0)
+		{
+			for (int i = 0; i <= m; ++i)
+				s_g[i] = 0.0;
+			s_g[0] = beta;
+		}
+		__threadfence_block();
+		__syncthreads();
+
+		int k = 0;
+		for (int j = 0; j < m; ++j)
+		{
+			T* __restrict__ Vj = V + (int64_t)j * nU;
+			T* __restrict__ W = V + (int64_t)(j + 1) * nU;
+			for (int64_t u = threadIdx.x; u < nU; u += pblock)
+				W[u] = (T)pg_apply_one<T>(S, n_seg, b, u, Vj, ib);
+			__threadfence_block();
+			__syncthreads();
+			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
+			for (int pass = 0; pass < 2; ++pass)
+			{
+				for (int i = 0; i <= j; ++i)
+				{
+					const T* __restrict__ Vi = V + (int64_t)i * nU;
+					double d = 0.0;
+					for (int64_t u = threadIdx.x; u < nU; u += pblock)
+						d += (double)W[u] * (double)Vi[u];
+					d = wave_sum(d);
+					if (lane_id() == 0)
+						s_part[i * pwaves + threadIdx.x / wave] = d;
+				}
+				__syncthreads();
+				if (threadIdx.x <= j)
+				{
+					double t = 0.0;
+					for (int w = 0; w < pwaves; ++w)
+						t += s_part[threadIdx.x * pwaves + w];
+					s_c[threadIdx.x] = t;
+					s_H[j * (pm_max + 1) + threadIdx.x] = (pass == 0 ? 0.0 : s_H[j * (pm_max + 1) + threadIdx.x]) + t;
+				}
+				__syncthreads();
+				double w2 = 0.0;
+				for (int64_t u = threadIdx.x; u < nU; u += pblock)
+				{
+					double w = (double)W[u];
+					for (int i = 0; i <= j; ++i)
+						w -= s_c[i] * (double)V[(int64_t)i * nU + u];
+					const T wt = (T)w;
+					W[u] = wt;
+					w2 += (double)wt * (double)wt;
+				}
+				w2 = pg_sum(w2, s_red);
+				if (pass == 1 && threadIdx.x == 0)
+					s_scalar[0] = sqrt(w2);
+				__threadfence_block();
+				__syncthreads();
+			}
+			// Givens rotations on column j (thread 0)
+			if (threadIdx.x == 0)
+			{
+				double* col = s_H + j * (pm_max + 1);
+				const double hn = s_scalar[0];
+				col[j + 1] = hn;
+				for (int i = 0; i < j; ++i)
+				{
+					const double a = col[i], c = col[i + 1];
+					col[i] = s_cs[i] * a + s_sn[i] * c;
+					col[i + 1] = -s_sn[i] * a + s_cs[i] * c;
+				}
+				const double a = col[j], c = col[j + 1];
+				const double rn = hypot(a, c);
+				const double cj = rn > 0.0 ? a / rn : 1.0, sj = rn > 0.0 ? c / rn : 0.0;
+				s_cs[j] = cj;
+				s_sn[j] = sj;
+				col[j] = rn;
+				col[j + 1] = 0.0;
+				s_g[j + 1] = -sj * s_g[j];
+				s_g[j] = cj * s_g[j];
+				s_scalar[1] = fabs(s_g[j + 1]) / bn;
+				s_scalar[2] = (hn > 0.0 && rn > 0.0) ? 1.0 / hn : 0.0;
+			}
+			__syncthreads();
+			const double scale = s_scalar[2];
+			for (int64_t u = threadIdx.x; u < nU; u += pblock)
+				W[u] = (T)((double)W[u] * scale);
+			__threadfence_block();
+			__syncthreads();
+			++iters;
+			k = j + 1;
+			if (s_scalar[1] <= 0.5 * tol || scale == 0.0 || iters >= max_iters)
+				break;
+		}
+		// y = H^-1 g, z += sum_i y_i V_i
+		if (threadIdx.x == 0)
+			for (int i = k - 1; i >= 0; --i)
+			{
+				double t = s_g[i];
+				for (int l = i + 1; l < k; ++l)
+					t -= s_H[l * (pm_max + 1) + i] * s_c[l];
+				const double d = s_H[i * (pm_max + 1) + i];
+				s_c[i] = d != 0.0 ? t / d : 0.0;
+			}
+		__syncthreads();
+		for (int64_t u = threadIdx.x; u < nU; u += pblock)
+		{
+			double acc = zb[u];
+			for (int i = 0; i < k; ++i)
+				acc += s_c[i] * (double)V[(int64_t)i * nU + u];
+			zb[u] = acc;
+		}
+		__threadfence_block();
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		out.iters[b] = iters;
+		out.cycles[b] = cycles;
+		out.res[b] = rel;
+	}
+}
+
 // ============================================================================
 // host driver
 
@@ -932,12 +1147,20 @@ struct gmres_t
 			for (int j = 0; j < m; ++j)
 			{
 				T* Vj = V.get() + (int64_t)j * vstride;
-				k_apply<T><<<grid, sblock, 0, c.stream>>>(S, n_seg, Vj, W.get());
+				// algorithmic bytes: V_j read, W written, 1/D read per unknown and set,
+				// plus the unknown's state, class and mask
+				timed_launch(c, "k_apply", (double)B * nU * (2.0 * sizeof(T) + 8.0) + 12.0 * nU,
+							 [&] { k_apply<T><<<grid, sblock, 0, c.stream>>>(S, n_seg, Vj, W.get()); });
 				// classical Gram-Schmidt, twice
-				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
+				timed_launch(c, "k_dots", dbytes, [&] {
+					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				});
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
-				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
-																 part.get());
+				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
+					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
+																	 part.get());
+				});
 				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
 				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
@@ -958,6 +1181,19 @@ struct gmres_t
 		}
 	}
 };
+
+// Systems up to this size per set run as one workgroup per set.  A single
+// large system (few sets) is faster spread over the whole device.
+int64_t per_set_max_unknowns(int B) { return B >= 64 ? 65536 : (B >= 8 ? 16384 : 4096); }
+
+// algorithmic bytes of one per-set GMRES launch, per iteration-free estimate:
+// structure once, then per set the rhs, z (read + write) and 1/D.
+double gmres_set_bytes(const sys_t& S, int m, int tbytes)
+{
+	(void)m;
+	(void)tbytes;
+	return (double)S.nU * 16.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
+}
 
 } // namespace
 
@@ -1150,7 +1386,54 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 		const double tol = a.opts.tol > 0 ? a.opts.tol : 1e-13;
 		const int max_iters = a.opts.max_iters > 0 ? a.opts.max_iters : 20000;
 		const int restart = std::min(std::max(a.opts.restart > 0 ? a.opts.restart : 32, 2), 64);
-		if (a.opts.precision == 1)
+		if (nU <= per_set_max_unknowns(B))
+		{
+			const int m = std::min(restart, pm_max);
+			dbuf<double> r;
+			dbuf<int32_t> its, cyc;
+			dbuf<double> res;
+			r.ensure((size_t)B * nU);
+			its.ensure(B);
+			cyc.ensure(B);
+			res.ensure(B);
+			pgm_out_t po{ its.get(), cyc.get(), res.get() };
+			const int nseg1 = std::max(n_seg, 1);
+			if (a.opts.precision == 1)
+			{
+				dbuf<float> V;
+				V.ensure((size_t)B * (m + 1) * nU);
+				timed_launch(c, "k_gmres_set<float>", gmres_set_bytes(S, m, 4), [&] {
+					k_gmres_set<float><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs.get(), z.get(), V.get(), r.get(), m,
+																   tol, max_iters, po);
+				});
+			}
+			else
+			{
+				dbuf<double> V;
+				V.ensure((size_t)B * (m + 1) * nU);
+				timed_launch(c, "k_gmres_set<double>", gmres_set_bytes(S, m, 8), [&] {
+					k_gmres_set<double><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs.get(), z.get(), V.get(), r.get(),
+																	m, tol, max_iters, po);
+				});
+			}
+			ESL_HIP(hipGetLastError());
+			std::vector<int32_t> h_its, h_cyc;
+			std::vector<double> h_res;
+			read_host(c, h_its, its.get(), B);
+			read_host(c, h_cyc, cyc.get(), B);
+			read_host(c, h_res, res.get(), B);
+			for (int b = 0; b < B; ++b)
+			{
+				resid = std::max(resid, h_res[b]);
+				iters = std::max(iters, h_its[b]);
+				cycles = std::max(cycles, h_cyc[b]);
+			}
+			if (!(resid <= tol))
+				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+											   std::to_string(resid) + " after " + std::to_string(iters) +
+											   " iterations");
+		}
+		else if (a.opts.precision == 1)
 		{
 			gmres_t<float> gm(c, S, std::max(n_seg, 1), restart);
 			gm.solve(rhs.get(), z.get(), tol, max_iters);

@@ -231,9 +231,12 @@ void table_build(ctx_t& c, graph_t& g, const model_t& m)
 	block_deg.ensure(nb + 1);
 
 	ESL_HIP(hipEventRecord(c.ev[0], c.stream));
-	k_flip_mask<<<nb, table_block, 0, c.stream>>>(N, n, g.rules.get(), g.rule_inputs.get(), g.rule_tables.get(),
-												  (int)tabs.size(), g.up_ok, g.down_ok, g.mask.get(),
-												  block_deg.get());
+	// algorithmic bytes: the 4-byte mask written per state
+	timed_launch(c, "k_flip_mask", 4.0 * n, [&] {
+		k_flip_mask<<<nb, table_block, 0, c.stream>>>(N, n, g.rules.get(), g.rule_inputs.get(), g.rule_tables.get(),
+													  (int)tabs.size(), g.up_ok, g.down_ok, g.mask.get(),
+													  block_deg.get());
+	});
 	k_scan_single<<<1, 1024, 0, c.stream>>>(block_deg.get(), nb);
 	ESL_HIP(hipGetLastError());
 	int32_t total = 0;
@@ -243,8 +246,11 @@ void table_build(ctx_t& c, graph_t& g, const model_t& m)
 	g.rows.ensure(total);
 	g.cols.ensure(total);
 	const size_t lds = sizeof(int32_t) * 2 * table_block * N;
-	k_table_fill<<<nb, table_block, lds, c.stream>>>(N, n, g.mask.get(), block_deg.get(), g.indptr.get(),
-													 g.rows.get(), g.cols.get());
+	// algorithmic bytes: mask read (4n), indptr (4(n+1)), rows and cols (4 nnz each) written
+	timed_launch(c, "k_table_fill", 4.0 * n + 4.0 * (n + 1) + 8.0 * (double)total, [&] {
+		k_table_fill<<<nb, table_block, lds, c.stream>>>(N, n, g.mask.get(), block_deg.get(), g.indptr.get(),
+														 g.rows.get(), g.cols.get());
+	});
 	ESL_HIP(hipGetLastError());
 	ESL_HIP(hipEventRecord(c.ev[1], c.stream));
 	ESL_HIP(hipEventSynchronize(c.ev[1]));

@@ -187,6 +187,24 @@ def test_single_rhs_equals_full_L():
         assert abs(a.sum() - 1) < 1e-12 and a.min() >= -1e-15
 
 
+def test_reachability_pruned_solve_equals_full():
+    # the pruned final-only oracle (used at N=20) against the full solve
+    import os
+
+    for f in sorted(os.listdir(model_path("database-small"))):
+        m = read_bnet(model_path("database-small/" + f))
+        if m.n_nodes > 12:
+            continue
+        rates = np.exp(np.random.default_rng(len(f) + 7).uniform(np.log(0.1), np.log(10), 2 * m.n_nodes))
+        n = 1 << m.n_nodes
+        x0 = P.initial_state(m, [m.nodes[0]], [1])
+        ip, rows, _ = P.transition_table(m, rates)
+        g = P.transition_graph(n, ip, rows)
+        a = P.solve(n, ip, rows, rates, x0, g, full=True).final_state
+        b = P.solve_final(n, ip, rows, rates, x0, g)
+        assert np.abs(a - b).sum() < 1e-12
+
+
 def test_parser_errors_and_grammar():
     with pytest.raises(BnetError):
         read_bnet_text("targets, factors\nA A\n")
