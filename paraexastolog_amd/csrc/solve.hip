@@ -270,6 +270,8 @@ struct sys_t
 	const double* rates;  // [B][2N]
 	const int32_t* anchor; // [B][n_seg] unknown index of the anchor of each segment
 	const double* invd;	   // [B][nU] 1/D' (1 for anchors)
+	int transpose;		   // 0: couple in-neighbours (flow of mass); 1: out-neighbours (absorption probabilities)
+	int64_t rate_stride;   // doubles between the rate vectors of consecutive sets (2N, or 0 when shared)
 };
 
 __device__ __forceinline__ double rate_of(const double* __restrict__ r, int i, bool up)
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(sblock) void k_invd(sys_t S, int n_seg, double* __r
 	if (cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u)
 		v = 1.0;
 	else
-		v = 1.0 / out_rate_s((uint32_t)s, S.mask[s], S.rates + (int64_t)b * 2 * S.N);
+		v = 1.0 / out_rate_s((uint32_t)s, S.mask[s], S.rates + (int64_t)b * S.rate_stride);
 	invd[(int64_t)b * S.nU + u] = v;
 }
 
@@ -312,7 +314,22 @@ __global__ __launch_bounds__(sblock) void k_invd(sys_t S, int n_seg, double* __r
 template <typename T, typename F>
 __device__ __forceinline__ void for_coupled(const sys_t& S, int b, uint32_t s, int32_t cls, F&& f)
 {
-	const double* __restrict__ r = S.rates + (int64_t)b * 2 * S.N;
+	const double* __restrict__ r = S.rates + (int64_t)b * S.rate_stride;
+	if (S.transpose)
+	{
+		// out-neighbours d of s: coefficient rate(s -> d), up iff bit i of s is 0
+		uint32_t m = S.mask[s];
+		while (m)
+		{
+			const int i = __ffs(m) - 1;
+			m &= m - 1;
+			const int32_t ud = S.pos[s ^ (1u << i)];
+			if (ud < 0)
+				continue;
+			f(ud, rate_of(r, i, ((s >> i) & 1u) == 0));
+		}
+		return;
+	}
 	for (int i = 0; i < S.N; ++i)
 	{
 		const uint32_t t = s ^ (1u << i);
@@ -617,7 +634,7 @@ __global__ __launch_bounds__(sblock) void k_anchor_sweep(sys_t S, const double* 
 	const int b = blockIdx.y;
 	if (u >= S.nUt)
 		return;
-	const double* r = S.rates + (int64_t)b * 2 * S.N;
+	const double* r = S.rates + (int64_t)b * S.rate_stride;
 	const int32_t s = S.ulist[u];
 	const int32_t cls = S.ucls[u];
 	const double* pb = p_in + (int64_t)b * S.nUt;
@@ -728,7 +745,7 @@ __global__ __launch_bounds__(sblock) void k_exit_flow(sys_t S, const int32_t* __
 	const int b = blockIdx.y;
 	if (u < S.nU)
 	{
-		const double* r = S.rates + (int64_t)b * 2 * S.N;
+		const double* r = S.rates + (int64_t)b * S.rate_stride;
 		const uint32_t t = (uint32_t)S.ulist[u];
 		const double vt = z[(int64_t)b * S.nU + u] * S.invd[(int64_t)b * S.nU + u];
 		uint32_t m = S.mask[t];
@@ -1347,6 +1364,8 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 	S.ucls = ucls.get();
 	S.pos = pos.get();
 	S.rates = a.d_rates;
+	S.rate_stride = 2 * N;
+	S.transpose = 0;
 	dbuf<int32_t> anchor;
 	anchor.ensure((size_t)B * std::max(n_seg, 1));
 	if (n_seg)
