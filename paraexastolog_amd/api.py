@@ -302,7 +302,41 @@ class solver:
                               C.byref(self.opts), C.byref(rep)), self.context.handle)
         self.final_state = out
         self.report = rep.as_dict()
+        self._solution = None
         return out
+
+    def _export_solution(self):
+        if getattr(self, "_solution", None) is None:
+            if not self.opts.full:
+                self.opts.full = 1
+                self.solve()
+            T = self.graph.terminals_count
+            n_t = int(self.graph.sccs_offsets[T])
+            ti = np.zeros(T + 1, dtype=np.int32)
+            tx = np.zeros(max(n_t, 1), dtype=np.int32)
+            td = np.zeros(max(n_t, 1))
+            nnz = C.c_int64()
+            check(lib().esl_solution_export(self.context.handle, self.table.handle, None, None, None, C.byref(nnz),
+                                            None, None, None), self.context.handle)
+            ni = np.zeros(T + 1, dtype=np.int32)
+            nx = np.zeros(max(nnz.value, 1), dtype=np.int32)
+            nd = np.zeros(max(nnz.value, 1))
+            check(lib().esl_solution_export(self.context.handle, self.table.handle, _ip(ti), _ip(tx), _dp(td), None,
+                                            _ip(ni), _ip(nx), _dp(nd)), self.context.handle)
+            self._solution = (sparse_cs_matrix(ti, tx[:n_t], td[:n_t]),
+                              sparse_cs_matrix(ni, nx[: nnz.value], nd[: nnz.value]))
+        return self._solution
+
+    @property
+    def solution_term(self):
+        """R: CSC n x T -- indptr = terminal offsets, indices = terminal vertices, data = pi."""
+        return self._export_solution()[0]
+
+    @property
+    def solution_nonterm(self):
+        """L: CSR T x n -- per terminal SCC its vertices (1) then the transient vertices
+        with a nonzero absorption probability, in canonical order."""
+        return self._export_solution()[1]
 
     def print_final_state(self, model_nodes, file=None):
         """Non-zero states as node names joined by ' --- ', '<nil>' for state 0,

@@ -86,6 +86,8 @@ void check_rates(const esl::graph_t& g, const double* rates)
 
 } // namespace
 
+esl::ctx_t& esl::ctx_impl(esl_ctx* ctx) { return ctx->c; }
+
 extern "C" {
 
 const char* esl_version(void) { return "paraexastolog-amd 0.1 (gfx950)"; }
@@ -400,6 +402,11 @@ esl_status esl_solve_batch_device(esl_ctx* ctx, const esl_graph* g, int n_sets, 
 		esl_solve_report rep{};
 		esl::solve_args_t a{ d_rates, d_x0, d_final, n_sets, o };
 		esl::solve_run(ctx->c, const_cast<esl::graph_t&>(g->g), a, rep);
+		if (o.full)
+		{
+			require(n_sets == 1, ESL_E_ARG, "opts.full needs a single rate set");
+			esl::solution_full(ctx->c, const_cast<esl::graph_t&>(g->g), d_rates, o);
+		}
 		if (report)
 			*report = rep;
 	});
@@ -435,6 +442,8 @@ esl_status esl_solve(esl_ctx* ctx, const esl_graph* g, const double* rates, cons
 		esl_solve_report rep{};
 		esl::solve_args_t a{ d_rates.get(), d_x0.get(), d_final.get(), 1, o };
 		esl::solve_run(c, const_cast<esl::graph_t&>(gg), a, rep);
+		if (o.full)
+			esl::solution_full(c, const_cast<esl::graph_t&>(gg), d_rates.get(), o);
 		ESL_HIP(hipMemcpyAsync(final_state, d_final.get(), gg.n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
 		ESL_HIP(hipStreamSynchronize(c.stream));
 		if (report)

@@ -224,6 +224,10 @@ struct solve_args_t
 	esl_solve_opts opts;
 };
 void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep);
+// the implementation object behind an esl_ctx handle (defined in esl_api.cpp)
+ctx_t& ctx_impl(esl_ctx* ctx);
+// solution_term / solution_nonterm of one rate set into g.sol (reference layout)
+void solution_full(ctx_t& c, graph_t& g, const double* d_rates, const esl_solve_opts& opts);
 
 // device scan helper: out[i] = sum_{j<i} in[j] (exclusive), out[n] = total; in/out int32
 void exclusive_scan_i32(ctx_t& c, const int32_t* in, int32_t* out, int64_t n);

@@ -1021,6 +1021,55 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 	}
 }
 
+
+// ============================================================================
+// full mode: terminal kernels of every terminal SCC and the absorption
+// probabilities X[j, c] of every transient state (solution_term /
+// solution_nonterm of solver.cu:102-300)
+
+__global__ __launch_bounds__(sblock) void k_fill_u8(int64_t n, uint8_t v, uint8_t* __restrict__ p)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+// rhs[j][u] = sum over edges c -> d leaving the ambiguous set of rate(c -> d)
+// when d is bound for terminal SCC j (transposed system, absorption probabilities)
+__global__ __launch_bounds__(sblock) void k_rhs_absorb(sys_t S, const int32_t* __restrict__ rank, int64_t T,
+													   const uint64_t* __restrict__ tm, double* __restrict__ rhs)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int j = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const double* r = S.rates;
+	const uint32_t c = (uint32_t)S.ulist[u];
+	uint32_t m = S.mask[c];
+	double acc = 0.0;
+	while (m)
+	{
+		const int i = __ffs(m) - 1;
+		m &= m - 1;
+		const uint32_t d = c ^ (1u << i);
+		if (S.pos[d] >= 0)
+			continue;
+		const int32_t k = rank[d];
+		const int t = k < T ? k : (__ffsll((unsigned long long)tm[d]) - 1);
+		if (t == j)
+			acc += rate_of(r, i, ((c >> i) & 1u) == 0);
+	}
+	rhs[(int64_t)j * S.nU + u] = acc;
+}
+
+__global__ __launch_bounds__(sblock) void k_times_invd(int64_t count, const double* __restrict__ z,
+													   const double* __restrict__ invd, double* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		out[i] = z[i] * invd[i];
+}
+
 // ============================================================================
 // host driver
 
@@ -1210,6 +1259,89 @@ double gmres_set_bytes(const sys_t& S, int m, int tbytes)
 	(void)m;
 	(void)tbytes;
 	return (double)S.nU * 16.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
+}
+
+// Solves A_p z = rhs for every set of S (z holds the initial guess): one
+// workgroup per set when the system is small, device-wide GMRES otherwise.
+struct krylov_result_t
+{
+	double resid = 0.0;
+	int iters = 0, cycles = 0;
+};
+
+krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
+							 const esl_solve_opts& opts)
+{
+	krylov_result_t kr;
+	const int64_t nU = S.nU;
+	const int B = S.B;
+	const double tol = opts.tol > 0 ? opts.tol : 1e-13;
+	const int max_iters = opts.max_iters > 0 ? opts.max_iters : 20000;
+	const int restart = std::min(std::max(opts.restart > 0 ? opts.restart : 32, 2), 64);
+	if (nU <= per_set_max_unknowns(B))
+	{
+		const int m = std::min(restart, pm_max);
+		dbuf<double> r;
+		dbuf<int32_t> its, cyc;
+		dbuf<double> res;
+		r.ensure((size_t)B * nU);
+		its.ensure(B);
+		cyc.ensure(B);
+		res.ensure(B);
+		pgm_out_t po{ its.get(), cyc.get(), res.get() };
+		const int nseg1 = std::max(n_seg, 1);
+		if (opts.precision == 1)
+		{
+			dbuf<float> V;
+			V.ensure((size_t)B * (m + 1) * nU);
+			timed_launch(c, "k_gmres_set<float>", gmres_set_bytes(S, m, 4), [&] {
+				k_gmres_set<float><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs_p, z_p, V.get(), r.get(), m, tol,
+															   max_iters, po);
+			});
+		}
+		else
+		{
+			dbuf<double> V;
+			V.ensure((size_t)B * (m + 1) * nU);
+			timed_launch(c, "k_gmres_set<double>", gmres_set_bytes(S, m, 8), [&] {
+				k_gmres_set<double><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs_p, z_p, V.get(), r.get(), m, tol,
+																max_iters, po);
+			});
+		}
+		ESL_HIP(hipGetLastError());
+		std::vector<int32_t> h_its, h_cyc;
+		std::vector<double> h_res;
+		read_host(c, h_its, its.get(), B);
+		read_host(c, h_cyc, cyc.get(), B);
+		read_host(c, h_res, res.get(), B);
+		for (int b = 0; b < B; ++b)
+		{
+			kr.resid = std::max(kr.resid, h_res[b]);
+			kr.iters = std::max(kr.iters, h_its[b]);
+			kr.cycles = std::max(kr.cycles, h_cyc[b]);
+		}
+		if (!(kr.resid <= tol))
+			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+										   std::to_string(kr.resid) + " after " + std::to_string(kr.iters) +
+										   " iterations");
+	}
+	else if (opts.precision == 1)
+	{
+		gmres_t<float> gm(c, S, std::max(n_seg, 1), restart);
+		gm.solve(rhs_p, z_p, tol, max_iters);
+		kr.resid = gm.final_res;
+		kr.iters = gm.iters;
+		kr.cycles = gm.cycles;
+	}
+	else
+	{
+		gmres_t<double> gm(c, S, std::max(n_seg, 1), restart);
+		gm.solve(rhs_p, z_p, tol, max_iters);
+		kr.resid = gm.final_res;
+		kr.iters = gm.iters;
+		kr.cycles = gm.cycles;
+	}
+	return kr;
 }
 
 } // namespace
@@ -1402,72 +1534,10 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 		k_rhs<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), a.d_x0, rhs.get());
 		k_zero<double><<<ceil_div((int64_t)B * nU, sblock), sblock, 0, c.stream>>>((int64_t)B * nU, z.get());
 		ESL_HIP(hipGetLastError());
-		const double tol = a.opts.tol > 0 ? a.opts.tol : 1e-13;
-		const int max_iters = a.opts.max_iters > 0 ? a.opts.max_iters : 20000;
-		const int restart = std::min(std::max(a.opts.restart > 0 ? a.opts.restart : 32, 2), 64);
-		if (nU <= per_set_max_unknowns(B))
-		{
-			const int m = std::min(restart, pm_max);
-			dbuf<double> r;
-			dbuf<int32_t> its, cyc;
-			dbuf<double> res;
-			r.ensure((size_t)B * nU);
-			its.ensure(B);
-			cyc.ensure(B);
-			res.ensure(B);
-			pgm_out_t po{ its.get(), cyc.get(), res.get() };
-			const int nseg1 = std::max(n_seg, 1);
-			if (a.opts.precision == 1)
-			{
-				dbuf<float> V;
-				V.ensure((size_t)B * (m + 1) * nU);
-				timed_launch(c, "k_gmres_set<float>", gmres_set_bytes(S, m, 4), [&] {
-					k_gmres_set<float><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs.get(), z.get(), V.get(), r.get(), m,
-																   tol, max_iters, po);
-				});
-			}
-			else
-			{
-				dbuf<double> V;
-				V.ensure((size_t)B * (m + 1) * nU);
-				timed_launch(c, "k_gmres_set<double>", gmres_set_bytes(S, m, 8), [&] {
-					k_gmres_set<double><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs.get(), z.get(), V.get(), r.get(),
-																	m, tol, max_iters, po);
-				});
-			}
-			ESL_HIP(hipGetLastError());
-			std::vector<int32_t> h_its, h_cyc;
-			std::vector<double> h_res;
-			read_host(c, h_its, its.get(), B);
-			read_host(c, h_cyc, cyc.get(), B);
-			read_host(c, h_res, res.get(), B);
-			for (int b = 0; b < B; ++b)
-			{
-				resid = std::max(resid, h_res[b]);
-				iters = std::max(iters, h_its[b]);
-				cycles = std::max(cycles, h_cyc[b]);
-			}
-			if (!(resid <= tol))
-				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-											   std::to_string(resid) + " after " + std::to_string(iters) +
-											   " iterations");
-		}
-		else if (a.opts.precision == 1)
-		{
-			gmres_t<float> gm(c, S, std::max(n_seg, 1), restart);
-			gm.solve(rhs.get(), z.get(), tol, max_iters);
-			resid = gm.final_res;
-			iters = gm.iters;
-			cycles = gm.cycles;
-		}
-		else
-		{
-			gmres_t<double> gm(c, S, std::max(n_seg, 1), restart);
-			gm.solve(rhs.get(), z.get(), tol, max_iters);
-			resid = gm.final_res;
-			iters = gm.iters;
-			cycles = gm.cycles;
-		}
+		const krylov_result_t kr = krylov_solve(c, S, n_seg, rhs.get(), z.get(), a.opts);
+		resid = kr.resid;
+		iters = kr.iters;
+		cycles = kr.cycles;
 	}
 	rep.residual = resid;
 	rep.iterations = iters;
@@ -1510,6 +1580,247 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 	for (int i = 0; i < 5; ++i)
 		tot += rep.ms_stage[i];
 	rep.ms_total = tot;
+}
+
+// solution_term and solution_nonterm in the reference layout (one rate set).
+void solution_full(ctx_t& c, graph_t& g, const double* d_rates, const esl_solve_opts& opts_in)
+{
+	esl_solve_opts opts = opts_in;
+	opts.precision = 0;
+	const int N = g.N;
+	const int64_t n = g.n, T = g.T, n_t = g.n_t;
+	const int nb = ceil_div(n, sblock);
+	flags_t fl(c);
+	solution_t& sol = g.sol;
+	sol = solution_t{};
+	std::vector<int32_t> h_reord;
+	read_host(c, h_reord, g.reordered.get(), (size_t)n);
+
+	// ---- (a) pi of every terminal SCC (row-replaced systems, no pruning)
+	std::vector<int32_t> seg_off(1, 0), seg_src, segmap(T, -1);
+	int n_seg = 0;
+	for (int64_t k = 0; k < T; ++k)
+	{
+		const int32_t sz = g.h_offsets[k + 1] - g.h_offsets[k];
+		if (sz >= 2)
+		{
+			segmap[k] = n_seg++;
+			seg_off.push_back(seg_off.back() + sz);
+			seg_src.push_back(g.h_offsets[k]);
+		}
+	}
+	sol.term_indptr.assign(g.h_offsets.begin(), g.h_offsets.begin() + T + 1);
+	sol.term_indices.assign(h_reord.begin(), h_reord.begin() + n_t);
+	sol.term_data.assign(n_t, 1.0);
+	dbuf<int32_t> pos;
+	pos.ensure(n);
+	if (n_seg)
+	{
+		const int64_t nUt = seg_off.back();
+		std::vector<int32_t> states(nUt), seg_of(nUt);
+		for (int k = 0; k < n_seg; ++k)
+			for (int32_t q = 0; q < seg_off[k + 1] - seg_off[k]; ++q)
+			{
+				states[seg_off[k] + q] = h_reord[seg_src[k] + q];
+				seg_of[seg_off[k] + q] = k;
+			}
+		dbuf<int32_t> ulist, ucls, d_seg_off, anchor;
+		ulist.ensure(nUt);
+		ucls.ensure(nUt);
+		d_seg_off.ensure(seg_off.size());
+		anchor.ensure(n_seg);
+		write_dev(c, ulist.get(), states);
+		write_dev(c, ucls.get(), seg_of);
+		write_dev(c, d_seg_off.get(), seg_off);
+		ESL_HIP(hipMemsetAsync(pos.get(), 0xff, n * sizeof(int32_t), c.stream));
+		dbuf<int32_t> d_states;
+		d_states.ensure(nUt);
+		write_dev(c, d_states.get(), states);
+		k_fill_term_unknowns<<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, d_states.get(), ucls.get(),
+																			 ulist.get(), ucls.get(), pos.get());
+		sys_t S{};
+		S.N = N;
+		S.nU = S.nUt = nUt;
+		S.B = 1;
+		S.mask = g.mask.get();
+		S.ulist = ulist.get();
+		S.ucls = ucls.get();
+		S.pos = pos.get();
+		S.rates = d_rates;
+		S.rate_stride = 2 * N;
+		S.transpose = 0;
+		dbuf<double> p0, p1;
+		p0.ensure(nUt);
+		p1.ensure(nUt);
+		write_dev(c, p0.get(), std::vector<double>(nUt, 1.0));
+		for (int it = 0; it < 32; ++it)
+		{
+			k_anchor_sweep<<<dim3(ceil_div(nUt, sblock), 1), sblock, 0, c.stream>>>(S, p0.get(), p1.get());
+			std::swap(p0, p1);
+		}
+		k_anchor_pick<<<dim3(n_seg, 1), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), p0.get(), n_seg, anchor.get());
+		S.anchor = anchor.get();
+		dbuf<double> invd, rhs, z, segsum, pi;
+		invd.ensure(nUt);
+		rhs.ensure(nUt);
+		z.ensure(nUt);
+		segsum.ensure(n_seg);
+		pi.ensure(nUt);
+		S.invd = invd.get();
+		const dim3 gu(ceil_div(nUt, sblock), 1);
+		k_invd<<<gu, sblock, 0, c.stream>>>(S, n_seg, invd.get());
+		k_rhs<<<gu, sblock, 0, c.stream>>>(S, n_seg, nullptr, rhs.get());
+		k_zero<double><<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, z.get());
+		ESL_HIP(hipGetLastError());
+		krylov_solve(c, S, n_seg, rhs.get(), z.get(), opts);
+		k_seg_sum<<<dim3(n_seg, 1), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), z.get(), invd.get(), n_seg,
+														  segsum.get());
+		k_times_invd<<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, z.get(), invd.get(), pi.get());
+		std::vector<double> h_pi, h_sum;
+		read_host(c, h_pi, pi.get(), nUt);
+		read_host(c, h_sum, segsum.get(), n_seg);
+		for (int k = 0; k < n_seg; ++k)
+			for (int32_t q = 0; q < seg_off[k + 1] - seg_off[k]; ++q)
+				sol.term_data[seg_src[k] + q] = h_pi[seg_off[k] + q] / h_sum[k];
+	}
+
+	// ---- (b) absorption probabilities X[j, c] of every transient state
+	sol.nonterm_indptr.assign(1, 0);
+	const int64_t n_nt = n - n_t;
+	if (n_nt == 0 || T == 1)
+	{
+		for (int64_t j = 0; j < T; ++j)
+		{
+			for (int32_t p = g.h_offsets[j]; p < g.h_offsets[j + 1]; ++p)
+			{
+				sol.nonterm_indices.push_back(h_reord[p]);
+				sol.nonterm_data.push_back(1.0);
+			}
+			for (int64_t p = n_t; p < n; ++p)
+			{
+				sol.nonterm_indices.push_back(h_reord[p]);
+				sol.nonterm_data.push_back(1.0);
+			}
+			sol.nonterm_indptr.push_back((int32_t)sol.nonterm_indices.size());
+		}
+		sol.valid = true;
+		return;
+	}
+	const bool fallback = T > max_rhs_terms;
+	if (T > 4096)
+		throw error(ESL_E_ARG, "full solution export supports at most 4096 terminal SCCs");
+	dbuf<uint8_t> reach;
+	reach.ensure(n);
+	k_fill_u8<<<nb, sblock, 0, c.stream>>>(n, 1, reach.get());
+	std::vector<int32_t> jmap(T);
+	for (int64_t k = 0; k < T; ++k)
+		jmap[k] = (int32_t)k;
+	dbuf<int32_t> d_jmap;
+	d_jmap.ensure(T);
+	write_dev(c, d_jmap.get(), jmap);
+	dbuf<uint64_t> tmask;
+	tmask.ensure(n);
+	k_tm_init<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, d_jmap.get(), reach.get(), tmask.get());
+	if (!fallback)
+		for (int p = 0;; ++p)
+		{
+			fl.clear(0);
+			k_tm<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), g.rank.get(), T, reach.get(), tmask.get(),
+											  fl.dev(0));
+			if (!fl.read(0))
+				break;
+			if (p > (1 << 24))
+				throw error(ESL_E_DEVICE, "terminal-mask propagation did not converge");
+		}
+	dbuf<int32_t> amb, fpos;
+	amb.ensure(n);
+	fpos.ensure(n + 1);
+	k_amb_flag<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, reach.get(), tmask.get(), fallback ? 1 : 0,
+											amb.get());
+	exclusive_scan_i32(c, amb.get(), fpos.get(), n);
+	int32_t nA = 0;
+	ESL_HIP(hipMemcpyAsync(&nA, fpos.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	std::vector<uint64_t> h_tm;
+	read_host(c, h_tm, tmask.get(), (size_t)n);
+	std::vector<int32_t> h_pos(n, -1);
+	std::vector<double> h_x; // [T][nA]
+	if (nA)
+	{
+		dbuf<int32_t> ulist, ucls;
+		ulist.ensure(nA);
+		ucls.ensure(nA);
+		ESL_HIP(hipMemsetAsync(pos.get(), 0xff, n * sizeof(int32_t), c.stream));
+		k_fill_unknowns<<<nb, sblock, 0, c.stream>>>(n, amb.get(), fpos.get(), 0, ulist.get(), ucls.get(),
+													 pos.get());
+		sys_t S{};
+		S.N = N;
+		S.nU = nA;
+		S.nUt = 0;
+		S.B = (int)T;
+		S.mask = g.mask.get();
+		S.ulist = ulist.get();
+		S.ucls = ucls.get();
+		S.pos = pos.get();
+		S.rates = d_rates;
+		S.rate_stride = 0; // every right-hand side shares the rates
+		S.transpose = 1;
+		dbuf<int32_t> anchor;
+		anchor.ensure(1);
+		S.anchor = anchor.get();
+		dbuf<double> invd, rhs, z;
+		invd.ensure((size_t)T * nA);
+		rhs.ensure((size_t)T * nA);
+		z.ensure((size_t)T * nA);
+		S.invd = invd.get();
+		const dim3 gu(ceil_div(nA, sblock), (unsigned)T);
+		k_invd<<<gu, sblock, 0, c.stream>>>(S, 1, invd.get());
+		k_rhs_absorb<<<gu, sblock, 0, c.stream>>>(S, g.rank.get(), T, tmask.get(), rhs.get());
+		k_zero<double><<<ceil_div((int64_t)T * nA, sblock), sblock, 0, c.stream>>>((int64_t)T * nA, z.get());
+		ESL_HIP(hipGetLastError());
+		krylov_solve(c, S, 0, rhs.get(), z.get(), opts);
+		dbuf<double> x;
+		x.ensure((size_t)T * nA);
+		k_times_invd<<<ceil_div((int64_t)T * nA, sblock), sblock, 0, c.stream>>>((int64_t)T * nA, z.get(), invd.get(),
+																				  x.get());
+		read_host(c, h_x, x.get(), (size_t)T * nA);
+		read_host(c, h_pos, pos.get(), (size_t)n);
+	}
+	std::vector<int32_t> h_rank;
+	read_host(c, h_rank, g.rank.get(), (size_t)n);
+	for (int64_t j = 0; j < T; ++j)
+	{
+		for (int32_t p = g.h_offsets[j]; p < g.h_offsets[j + 1]; ++p)
+		{
+			sol.nonterm_indices.push_back(h_reord[p]);
+			sol.nonterm_data.push_back(1.0);
+		}
+		for (int64_t p = n_t; p < n; ++p)
+		{
+			const int32_t cst = h_reord[p];
+			const int32_t u = h_pos[cst];
+			double v;
+			if (u >= 0)
+			{
+				if (!fallback && !((h_tm[cst] >> j) & 1ull))
+					continue;
+				v = h_x[(size_t)j * nA + u];
+				if (v == 0.0)
+					continue;
+			}
+			else
+			{
+				const int t = __builtin_ffsll((long long)h_tm[cst]) - 1;
+				if (t != j)
+					continue;
+				v = 1.0;
+			}
+			sol.nonterm_indices.push_back(cst);
+			sol.nonterm_data.push_back(v);
+		}
+		sol.nonterm_indptr.push_back((int32_t)sol.nonterm_indices.size());
+	}
+	sol.valid = true;
 }
 
 } // namespace esl

@@ -158,3 +158,37 @@ def test_synthetic_vs_oracle(ctx, cfg, seed):
         x = s.final_state
         assert abs(x.sum() - 1) < 1e-12 and x.min() >= -1e-14
         assert s.report["residual"] <= 1e-13
+
+
+@pytest.mark.parametrize("name", ["solver.toy", "solver.toy2", "solver.toy3"])
+def test_solution_export_goldens(ctx, name):
+    # solution_term (CSC) and solution_nonterm (CSR) in the reference layout (test.cu:101-111)
+    c, t, g, s = run_case(ctx, name, full=1)
+    R, L = s.solution_term, s.solution_nonterm
+    assert R.indptr.tolist() == c["term_indptr"]
+    assert R.indices.tolist() == c["term_indices"]
+    np.testing.assert_allclose(R.data, c["term_data"], rtol=0, atol=1e-14)
+    assert L.indptr.tolist() == c["nonterm_indptr"]
+    assert L.indices.tolist() == c["nonterm_indices"]
+    np.testing.assert_allclose(L.data, c["nonterm_data"], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["toy-paper.bnet", "database-small/007.bnet", "database-small/031.bnet",
+                                  "database-small/088.bnet", "database-small/110.bnet", "database-small/158.bnet"])
+def test_solution_export_vs_oracle(ctx, name):
+    om = read_bnet(model_path(name))
+    rates = np.exp(np.random.default_rng(len(name) + 1).uniform(np.log(0.1), np.log(10), 2 * om.n_nodes))
+    pm = model_builder().construct_model(model_path(name))
+    t, g = build(ctx, pm, rates)
+    s = solver(ctx, t, g, rates, initial_state(pm.nodes), full=1)
+    s.solve()
+    n, ip, rows, cols, graph = oracle_graph(om, rates)
+    ref = P.solve(n, ip, rows, rates, P.initial_state(om), graph, full=True)
+    R, L = s.solution_term, s.solution_nonterm
+    np.testing.assert_array_equal(R.indptr, ref.term_indptr)
+    np.testing.assert_array_equal(R.indices, ref.term_indices)
+    np.testing.assert_allclose(R.data, ref.term_data, rtol=1e-10, atol=1e-13)
+    np.testing.assert_array_equal(L.indptr, ref.nonterm_indptr)
+    np.testing.assert_array_equal(L.indices, ref.nonterm_indices)
+    np.testing.assert_allclose(L.data, ref.nonterm_data, rtol=1e-10, atol=1e-13)
+    assert l1(s.final_state, ref.final_state) <= 1e-12
