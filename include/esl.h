@@ -135,7 +135,9 @@ typedef struct
 	int max_iters;		 /* Krylov iterations per solve before ESL_E_NUMERIC (default 20000) */
 	int restart;		 /* GMRES restart length (default 32) */
 	int full;			 /* 1: also build solution_term / solution_nonterm (esl_solution_export) */
-	const double* warm_pi; /* optional warm start of the terminal kernels (length n, may be NULL) */
+	const double* warm_pi; /* optional warm start of the terminal kernels: predicted final-vector values
+							* on the terminal states in canonical order (reordered_vertices[0, n_t)),
+							* [n_sets][n_t]; host memory for esl_solve, device memory otherwise */
 } esl_solve_opts;
 
 typedef struct
@@ -146,6 +148,7 @@ typedef struct
 	int64_t reachable;	 /* states reachable from supp(x0) */
 	int64_t unknowns;	 /* unknowns of the linear systems (terminal kernels + ambiguous transient states) */
 	int terminals_reached; /* terminal SCCs reachable from supp(x0) */
+	int sets_iterated;	 /* rate sets whose initial iterate (zero, or the warm start) missed the tolerance */
 	double ms_total;	 /* device time of the call */
 	double ms_stage[8];	 /* reach, terminal masks, setup, terminal solve, transient solve, final, -, - */
 } esl_solve_report;

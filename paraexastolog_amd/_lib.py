@@ -41,6 +41,7 @@ class SolveReport(C.Structure):
         ("reachable", C.c_int64),
         ("unknowns", C.c_int64),
         ("terminals_reached", C.c_int),
+        ("sets_iterated", C.c_int),
         ("ms_total", C.c_double),
         ("ms_stage", C.c_double * 8),
     ]
@@ -52,12 +53,13 @@ class SolveReport(C.Structure):
             "refinements": self.refinements,
             "unknowns": self.unknowns,
             "terminals_reached": self.terminals_reached,
+            "sets_iterated": self.sets_iterated,
             "ms_total": self.ms_total,
             "ms_stage": list(self.ms_stage)[:5],
         }
 
 
-# (name, restype, argtypes) of every entry point in include/esl.h
+# (name, restype, argtypes) of every entry point in include/esl.h and include/esl_mlp.h
 _VP = C.c_void_p
 _P = C.POINTER
 SIGNATURES = [
@@ -93,6 +95,14 @@ SIGNATURES = [
     ("esl_memcpy_d2h", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_graph_stage_ms", C.c_int, [_VP, _P(C.c_double)]),
     ("esl_profile", C.c_int, [_VP, C.c_int]),
+    ("esl_mlp_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _P(_VP)]),
+    ("esl_mlp_destroy", None, [_VP]),
+    ("esl_mlp_num_params", C.c_int64, [_VP]),
+    ("esl_mlp_get_params", C.c_int, [_VP, _VP, _P(C.c_float)]),
+    ("esl_mlp_set_params", C.c_int, [_VP, _VP, _P(C.c_float)]),
+    ("esl_mlp_forward", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
+    ("esl_mlp_gradients", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(C.c_float), _P(C.c_float)]),
+    ("esl_mlp_train_step", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.c_float, _P(C.c_float)]),
     ("esl_profile_stat", C.c_int, [_VP, C.c_int, C.c_char_p, C.c_int, _P(C.c_int64), _P(C.c_double),
                                    _P(C.c_double)]),
 ]

@@ -439,6 +439,15 @@ esl_status esl_solve(esl_ctx* ctx, const esl_graph* g, const double* rates, cons
 		if (opts)
 			o = *opts;
 		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		esl::dbuf<double> d_warm;
+		if (o.warm_pi)
+		{
+			require(gg.reorganized, ESL_E_STATE, "esl_graph_reorganize must run before a solve");
+			d_warm.ensure(std::max<int64_t>(gg.n_t, 1));
+			ESL_HIP(hipMemcpyAsync(d_warm.get(), o.warm_pi, gg.n_t * sizeof(double), hipMemcpyHostToDevice,
+								   c.stream));
+			o.warm_pi = d_warm.get();
+		}
 		esl_solve_report rep{};
 		esl::solve_args_t a{ d_rates.get(), d_x0.get(), d_final.get(), 1, o };
 		esl::solve_run(c, const_cast<esl::graph_t&>(gg), a, rep);

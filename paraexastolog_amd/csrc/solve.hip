@@ -1070,6 +1070,35 @@ __global__ __launch_bounds__(sblock) void k_times_invd(int64_t count, const doub
 		out[i] = z[i] * invd[i];
 }
 
+
+// warm start: p[b][u] = predicted mass of the terminal-system unknown u
+// (warm holds [set][n_t] values over the terminal states in canonical order)
+__global__ __launch_bounds__(sblock) void k_warm_gather(int64_t nUt, int64_t n_t, const int32_t* __restrict__ tpos,
+														const double* __restrict__ warm, double* __restrict__ p)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= nUt)
+		return;
+	const double v = warm[(int64_t)b * n_t + tpos[u]];
+	p[(int64_t)b * nUt + u] = v > 0.0 ? v : 0.0;
+}
+
+// initial iterate z = D' x with x_u = p_u / p_anchor on the terminal systems
+__global__ __launch_bounds__(sblock) void k_warm_z(sys_t S, int n_seg, const double* __restrict__ p,
+												   double* __restrict__ z)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nUt)
+		return;
+	const int32_t k = S.ucls[u];
+	const int32_t a = S.anchor[(int64_t)b * n_seg + k];
+	const double pa = p[(int64_t)b * S.nUt + a];
+	const double x = pa > 0.0 ? p[(int64_t)b * S.nUt + u] / pa : (u == a ? 1.0 : 0.0);
+	z[(int64_t)b * S.nU + u] = x / S.invd[(int64_t)b * S.nU + u];
+}
+
 // ============================================================================
 // host driver
 
@@ -1267,6 +1296,7 @@ struct krylov_result_t
 {
 	double resid = 0.0;
 	int iters = 0, cycles = 0;
+	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
 };
 
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
@@ -1319,6 +1349,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 			kr.resid = std::max(kr.resid, h_res[b]);
 			kr.iters = std::max(kr.iters, h_its[b]);
 			kr.cycles = std::max(kr.cycles, h_cyc[b]);
+			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
 		}
 		if (!(kr.resid <= tol))
 			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
@@ -1332,6 +1363,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		kr.resid = gm.final_res;
 		kr.iters = gm.iters;
 		kr.cycles = gm.cycles;
+		kr.sets_iterated = gm.cycles > 0 ? B : 0;
 	}
 	else
 	{
@@ -1340,6 +1372,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		kr.resid = gm.final_res;
 		kr.iters = gm.iters;
 		kr.cycles = gm.cycles;
+		kr.sets_iterated = gm.cycles > 0 ? B : 0;
 	}
 	return kr;
 }
@@ -1441,7 +1474,7 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 	const int64_t nUt = seg_off.back();
 	const int64_t nU = nUt + nA;
 	rep.unknowns = nU;
-	dbuf<int32_t> pos, ulist, ucls, d_seg_off;
+	dbuf<int32_t> pos, ulist, ucls, d_seg_off, d_tpos;
 	pos.ensure(n);
 	ulist.ensure(nU + 1);
 	ucls.ensure(nU + 1);
@@ -1453,13 +1486,16 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 		// states of the segments, from the canonical order
 		std::vector<int32_t> h_reord;
 		read_host(c, h_reord, g.reordered.get(), (size_t)n_t);
-		std::vector<int32_t> states(nUt), seg_of(nUt);
+		std::vector<int32_t> states(nUt), seg_of(nUt), tpos(nUt);
 		for (int k = 0; k < n_seg; ++k)
 			for (int32_t q = 0; q < seg_off[k + 1] - seg_off[k]; ++q)
 			{
 				states[seg_off[k] + q] = h_reord[seg_src[k] + q];
 				seg_of[seg_off[k] + q] = k;
+				tpos[seg_off[k] + q] = seg_src[k] + q;
 			}
+		d_tpos.ensure(nUt);
+		write_dev(c, d_tpos.get(), tpos);
 		dbuf<int32_t> d_states, d_segof;
 		d_states.ensure(nUt);
 		d_segof.ensure(nUt);
@@ -1500,18 +1536,25 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 	S.transpose = 0;
 	dbuf<int32_t> anchor;
 	anchor.ensure((size_t)B * std::max(n_seg, 1));
+	dbuf<double> p0, p1;
+	const bool warm = a.opts.warm_pi != nullptr && n_seg > 0;
 	if (n_seg)
 	{
-		dbuf<double> p0, p1;
 		p0.ensure((size_t)B * nUt);
 		p1.ensure((size_t)B * nUt);
-		std::vector<double> ones((size_t)B * nUt, 1.0);
-		write_dev(c, p0.get(), ones);
 		const dim3 gt(ceil_div(nUt, sblock), B);
-		for (int it = 0; it < 32; ++it)
+		if (warm)
+			// the prediction picks the anchors (no Jacobi sweeps)
+			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, d_tpos.get(), a.opts.warm_pi, p0.get());
+		else
 		{
-			k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, p0.get(), p1.get());
-			std::swap(p0, p1);
+			std::vector<double> ones((size_t)B * nUt, 1.0);
+			write_dev(c, p0.get(), ones);
+			for (int it = 0; it < 32; ++it)
+			{
+				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, p0.get(), p1.get());
+				std::swap(p0, p1);
+			}
 		}
 		k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), p0.get(), n_seg, anchor.get());
 		ESL_HIP(hipGetLastError());
@@ -1533,11 +1576,14 @@ void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& re
 		k_invd<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), invd.get());
 		k_rhs<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), a.d_x0, rhs.get());
 		k_zero<double><<<ceil_div((int64_t)B * nU, sblock), sblock, 0, c.stream>>>((int64_t)B * nU, z.get());
+		if (warm)
+			k_warm_z<<<dim3(ceil_div(nUt, sblock), B), sblock, 0, c.stream>>>(S, n_seg, p0.get(), z.get());
 		ESL_HIP(hipGetLastError());
 		const krylov_result_t kr = krylov_solve(c, S, n_seg, rhs.get(), z.get(), a.opts);
 		resid = kr.resid;
 		iters = kr.iters;
 		cycles = kr.cycles;
+		rep.sets_iterated = kr.sets_iterated;
 	}
 	rep.residual = resid;
 	rep.iterations = iters;

@@ -19,11 +19,11 @@ from paraexastolog_amd._lib import LIB_PATH, SIGNATURES, lib
 
 from .helpers import MODELS, REAL_MODELS, ROOT, SMALL_DB, golden, model_path
 
-HEADER = os.path.join(ROOT, "include", "esl.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("esl.h", "esl_mlp.h")]
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(esl_[a-z0-9_]+)\s*\(", text)))
 
