@@ -319,6 +319,20 @@ def solve(n, indptr, rows, rates, x0, graph, full=True):
             A = -(U @ B)  # T x n_nt
             X = sla.spsolve(Nm.T.tocsc(), A.T.toarray() if T > 1 else A.T.toarray().ravel())
             X = np.asarray(X).reshape(len(ntv), T).T if T > 1 else np.asarray(X).reshape(1, -1)
+            # X[j, c] is zero exactly when c cannot reach terminal SCC j; the sparse LU
+            # can leave rounding noise there, the reference's triangular solves do not
+            cols_ = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+            At = sp.csr_matrix((np.ones(len(rows), dtype=np.int8), (rows, cols_)), shape=(n, n))
+            for j in range(T):
+                seen = np.zeros(n, dtype=bool)
+                seen[reordered[offsets[j]:offsets[j + 1]]] = True
+                frontier = seen.copy()
+                while frontier.any():
+                    nxt = (At.T @ frontier.astype(np.int8)) > 0
+                    new = nxt & ~seen
+                    seen |= new
+                    frontier = new
+                X[j, ~seen[ntv]] = 0.0
             ip = [0]
             ind, dat = [], []
             for j in range(T):
