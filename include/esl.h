@@ -135,6 +135,7 @@ typedef struct
 	int max_iters;		 /* Krylov iterations per solve before ESL_E_NUMERIC (default 20000) */
 	int restart;		 /* GMRES restart length (default 32) */
 	int full;			 /* 1: also build solution_term / solution_nonterm (esl_solution_export) */
+	int preconditioner;	 /* 0 = default (ILU(0)), 1 = Jacobi, 2 = ILU(0) (level-scheduled) */
 	const double* warm_pi; /* optional warm start of the terminal kernels: predicted final-vector values
 							* on the terminal states in canonical order (reordered_vertices[0, n_t)),
 							* [n_sets][n_t]; host memory for esl_solve, device memory otherwise */

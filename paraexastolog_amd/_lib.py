@@ -29,6 +29,7 @@ class SolveOpts(C.Structure):
         ("max_iters", C.c_int),
         ("restart", C.c_int),
         ("full", C.c_int),
+        ("preconditioner", C.c_int),
         ("warm_pi", C.POINTER(C.c_double)),
     ]
 

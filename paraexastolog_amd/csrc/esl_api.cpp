@@ -383,6 +383,7 @@ void esl_solve_opts_default(esl_solve_opts* o)
 	o->max_iters = 20000;
 	o->restart = 32;
 	o->full = 0;
+	o->preconditioner = 0;
 	o->warm_pi = nullptr;
 }
 

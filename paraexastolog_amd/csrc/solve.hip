@@ -810,22 +810,131 @@ __global__ __launch_bounds__(sblock) void k_y_total(int B, int Tr, const double*
 
 
 // ============================================================================
+// Compressed coupling structure of the systems (shared by every set): for each
+// unknown u the coupled neighbours k -- in-neighbours for the forward (mass
+// flow) system, out-neighbours for the transposed (absorption) one -- with
+// the rate index of the off-diagonal entry M[u, k] (bits 0..5) and a flag
+// (bit 6) telling whether the mirrored entry M[k, u] exists.  Neighbours of
+// smaller state come first (ILU lower part).  M[u, k] = -rates[idx]; the
+// mirrored entry uses rates[idx ^ 1].
+
+constexpr int nbr_rev_flag = 64;
+
+__device__ __forceinline__ int coupled_list(const sys_t& S, int64_t u, int32_t* out_u, uint8_t* out_r, int& n_low)
+{
+	const uint32_t s = (uint32_t)S.ulist[u];
+	const int32_t cls = S.ucls[u];
+	int cnt = 0;
+	n_low = 0;
+	for (int pass = 0; pass < 2; ++pass) // 0: neighbours below s, 1: above
+		for (int i = 0; i < S.N; ++i)
+		{
+			const bool below = (s >> i) & 1u; // s ^ (1 << i) < s
+			if (below != (pass == 0))
+				continue;
+			const uint32_t k = s ^ (1u << i);
+			bool edge, rev;
+			int idx;
+			if (S.transpose)
+			{
+				edge = (S.mask[s] >> i) & 1u; // u -> k
+				rev = (S.mask[k] >> i) & 1u;
+				idx = 2 * i + (below ? 1 : 0); // u -> k is a down flip when k < s
+			}
+			else
+			{
+				edge = (S.mask[k] >> i) & 1u; // k -> u
+				rev = (S.mask[s] >> i) & 1u;
+				idx = 2 * i + (below ? 0 : 1); // k -> u is an up flip when k < s
+			}
+			if (!edge)
+				continue;
+			const int32_t uk = S.pos[k];
+			if (uk < 0)
+				continue;
+			if (!S.transpose && ((S.ucls[uk] < 0) != (cls < 0)))
+				continue;
+			if (out_u)
+			{
+				out_u[cnt] = uk;
+				out_r[cnt] = (uint8_t)(idx | (rev ? nbr_rev_flag : 0));
+			}
+			++cnt;
+			if (pass == 0)
+				++n_low;
+		}
+	return cnt;
+}
+
+__global__ __launch_bounds__(sblock) void k_nbr_count(sys_t S, int32_t* __restrict__ cnt)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (u >= S.nU)
+		return;
+	int lo;
+	cnt[u] = coupled_list(S, u, nullptr, nullptr, lo);
+}
+
+__global__ __launch_bounds__(sblock) void k_nbr_fill(sys_t S, const int32_t* __restrict__ ptr,
+													 int32_t* __restrict__ nbr_u, uint8_t* __restrict__ nbr_r,
+													 int32_t* __restrict__ mid, int32_t* __restrict__ lvl_key)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (u >= S.nU)
+		return;
+	int lo;
+	const int32_t b = ptr[u];
+	coupled_list(S, u, nbr_u + b, nbr_r + b, lo);
+	mid[u] = b + lo;
+	lvl_key[u] = __popc((uint32_t)S.ulist[u]);
+}
+
+struct levels_t
+{
+	int n; // levels (N + 1)
+	int off[34];
+};
+
+struct nbr_t
+{
+	const int32_t* ptr; // nU + 1
+	const int32_t* mid; // end of the lower part of each row
+	const int32_t* nu;	// neighbour unknowns
+	const uint8_t* nr;	// rate index | mirrored flag
+	const int32_t* lvl; // unknowns grouped by popcount of their state
+	levels_t L;
+};
+
+// ============================================================================
 // One workgroup per rate set: the whole restarted GMRES of that set in one
 // launch.  For systems up to a few tens of thousands of unknowns per set this
 // keeps every reduction inside the workgroup (no partial sums through HBM, no
 // host round trip per iteration); a batch of rate sets is one launch of
-// n_sets workgroups.  The Krylov basis lives in global memory (it stays in
-// L2); the Hessenberg matrix and the rotations live in LDS.
+// n_sets workgroups.  The iterate x (FP64) is unpreconditioned; right
+// preconditioning by Jacobi or by ILU(0).  ILU(0) of these matrices has the
+// pattern of M and, ordered by state, dependencies only between states whose
+// popcounts differ by one: its triangular solves are level-scheduled over the
+// N + 1 popcount levels (one workgroup barrier per level).  The Krylov basis
+// lives in global memory (it stays in L2); the Hessenberg matrix and the
+// rotations live in LDS.
 
 constexpr int pblock = 256;
 constexpr int pwaves = pblock / wave;
-constexpr int pm_max = 32;
+constexpr int pm_max = 64;
 
 struct pgm_out_t
 {
 	int32_t* iters;	  // [B]
 	int32_t* cycles;  // [B]
 	double* res;	  // [B] final relative FP64 residual
+};
+
+struct pgm_scratch_t
+{
+	double* r;	   // [B][nU] residual / FP64 correction
+	double* udiag; // [B][nU] ILU(0) pivots
+	void* t;	   // [B][nU] preconditioned basis vector (basis precision)
+	void* V;	   // [B][m+1][nU]
 };
 
 // sum over the workgroup of one value per thread, result broadcast
@@ -842,50 +951,135 @@ __device__ __forceinline__ double pg_sum(double v, double* lds)
 	return t;
 }
 
-template <typename T>
-__device__ __forceinline__ double pg_apply_one(const sys_t& S, int n_seg, int b, int64_t u, const T* __restrict__ vb,
-											   const double* __restrict__ ib)
+__device__ __forceinline__ bool is_anchor(const sys_t& S, int n_seg, int b, int64_t u)
 {
-	const int32_t s = S.ulist[u];
 	const int32_t cls = S.ucls[u];
-	double acc = (double)vb[u];
-	const bool anchor = cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u;
-	if (!anchor)
-		for_coupled<T>(S, b, (uint32_t)s, cls, [&](int32_t ut, double rate) { acc -= rate * ib[ut] * (double)vb[ut]; });
+	return cls >= 0 && S.anchor[(int64_t)b * n_seg + cls] == (int32_t)u;
+}
+
+// y = M x for one row (FP64): D'_u x_u - sum rate x_k; anchor rows give x_u
+template <typename X>
+__device__ __forceinline__ double row_apply(const sys_t& S, const nbr_t& G, int n_seg, int b, int64_t u,
+											const X* __restrict__ x, const double* __restrict__ r)
+{
+	if (is_anchor(S, n_seg, b, u))
+		return (double)x[u];
+	double acc = (double)x[u] / S.invd[(int64_t)b * S.nU + u];
+	for (int32_t e = G.ptr[u]; e < G.ptr[u + 1]; ++e)
+		acc -= r[G.nr[e] & 63] * (double)x[G.nu[e]];
 	return acc;
 }
 
-template <typename T>
-__global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const double* __restrict__ rhs,
-													  double* __restrict__ z, T* __restrict__ Vall,
-													  double* __restrict__ rall, int m, double tol, int max_iters,
-													  pgm_out_t out)
+// in place: v <- P^{-1} v with the ILU(0) factors (level-scheduled sweeps)
+template <typename X>
+__device__ void ilu_apply(const sys_t& S, const nbr_t& G, int n_seg, int b, const double* __restrict__ ud,
+						  const double* __restrict__ r, X* __restrict__ v)
+{
+	for (int l = 0; l < G.L.n; ++l)
+	{
+		const int a = G.L.off[l], e = G.L.off[l + 1];
+		if (a == e)
+			continue;
+		for (int q = a + threadIdx.x; q < e; q += pblock)
+		{
+			const int32_t u = G.lvl[q];
+			if (is_anchor(S, n_seg, b, u))
+				continue; // y_a = v_a
+			double y = (double)v[u];
+			for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
+			{
+				const int32_t uk = G.nu[k];
+				y += r[G.nr[k] & 63] / ud[uk] * (double)v[uk];
+			}
+			v[u] = (X)y;
+		}
+		__threadfence_block();
+		__syncthreads();
+	}
+	for (int l = G.L.n - 1; l >= 0; --l)
+	{
+		const int a = G.L.off[l], e = G.L.off[l + 1];
+		if (a == e)
+			continue;
+		for (int q = a + threadIdx.x; q < e; q += pblock)
+		{
+			const int32_t u = G.lvl[q];
+			if (is_anchor(S, n_seg, b, u))
+				continue; // z_a = y_a / 1
+			double z = (double)v[u];
+			for (int32_t k = G.mid[u]; k < G.ptr[u + 1]; ++k)
+				z += r[G.nr[k] & 63] * (double)v[G.nu[k]];
+			v[u] = (X)(z / ud[u]);
+		}
+		__threadfence_block();
+		__syncthreads();
+	}
+}
+
+template <typename T, bool ILU>
+__global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
+													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
+													  int max_iters, pgm_out_t out)
 {
 	__shared__ double s_H[(pm_max + 1) * pm_max];
 	__shared__ double s_cs[pm_max], s_sn[pm_max], s_g[pm_max + 1], s_c[pm_max + 1];
 	__shared__ double s_part[(pm_max + 1) * pwaves];
 	__shared__ double s_red[pwaves];
 	__shared__ double s_scalar[4];
+	__shared__ double s_rates[64];
 
 	const int b = blockIdx.x;
 	const int64_t nU = S.nU;
 	const int64_t off = (int64_t)b * nU;
 	const double* __restrict__ rb = rhs + off;
-	double* __restrict__ zb = z + off;
-	double* __restrict__ r = rall + off;
+	double* __restrict__ x = zio + off; // holds z = D' x on entry and exit, x inside
+	double* __restrict__ rr_ = sc.r + off;
+	double* __restrict__ ud = sc.udiag + off;
 	const double* __restrict__ ib = S.invd + off;
-	T* __restrict__ V = Vall + (int64_t)b * (m + 1) * nU;
+	T* __restrict__ t = (T*)sc.t + off;
+	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
+	for (int i = threadIdx.x; i < 2 * S.N; i += pblock)
+		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
+	for (int64_t u = threadIdx.x; u < nU; u += pblock)
+		x[u] *= ib[u];
+	__syncthreads();
+
+	if (ILU)
+	{
+		// pivots: u_ss = D'_s - sum over lower k with a mirrored entry of M[s,k] M[k,s] / u_kk
+		for (int l = 0; l < G.L.n; ++l)
+		{
+			const int a = G.L.off[l], e = G.L.off[l + 1];
+			if (a == e)
+				continue;
+			for (int q = a + threadIdx.x; q < e; q += pblock)
+			{
+				const int32_t u = G.lvl[q];
+				double d = 1.0 / ib[u];
+				if (!is_anchor(S, n_seg, b, u))
+					for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
+					{
+						const int code = G.nr[k];
+						const int32_t uk = G.nu[k];
+						if ((code & nbr_rev_flag) && !is_anchor(S, n_seg, b, uk))
+							d -= s_rates[code & 63] * s_rates[(code & 63) ^ 1] / ud[uk];
+					}
+				ud[u] = d;
+			}
+			__threadfence_block();
+			__syncthreads();
+		}
+	}
+
 	int iters = 0, cycles = 0;
 	double rel = 0.0;
-
 	for (;;)
 	{
-		// FP64 residual of the current iterate
 		double rr = 0.0, bb = 0.0;
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
 		{
-			const double res = rb[u] - pg_apply_one<double>(S, n_seg, b, u, zb, ib);
-			r[u] = res;
+			const double res = rb[u] - row_apply<double>(S, G, n_seg, b, u, x, s_rates);
+			rr_[u] = res;
 			rr += res * res;
 			bb += rb[u] * rb[u];
 		}
@@ -899,7 +1093,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 		++cycles;
 		const double inv_beta = 1.0 / beta;
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
-			V[u] = (T)(r[u] * inv_beta);
+			V[u] = (T)(rr_[u] * inv_beta);
 		if (threadIdx.x == 0)
 		{
 			for (int i = 0; i <= m; ++i)
@@ -912,10 +1106,26 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 		int k = 0;
 		for (int j = 0; j < m; ++j)
 		{
-			T* __restrict__ Vj = V + (int64_t)j * nU;
+			const T* __restrict__ Vj = V + (int64_t)j * nU;
 			T* __restrict__ W = V + (int64_t)(j + 1) * nU;
+			// t = P^{-1} V_j
+			if (ILU)
+			{
+				for (int64_t u = threadIdx.x; u < nU; u += pblock)
+					t[u] = Vj[u];
+				__threadfence_block();
+				__syncthreads();
+				ilu_apply<T>(S, G, n_seg, b, ud, s_rates, t);
+			}
+			else
+			{
+				for (int64_t u = threadIdx.x; u < nU; u += pblock)
+					t[u] = (T)((double)Vj[u] * ib[u]);
+				__threadfence_block();
+				__syncthreads();
+			}
 			for (int64_t u = threadIdx.x; u < nU; u += pblock)
-				W[u] = (T)pg_apply_one<T>(S, n_seg, b, u, Vj, ib);
+				W[u] = (T)row_apply<T>(S, G, n_seg, b, u, t, s_rates);
 			__threadfence_block();
 			__syncthreads();
 			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
@@ -934,11 +1144,11 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 				__syncthreads();
 				if (threadIdx.x <= j)
 				{
-					double t = 0.0;
+					double tt = 0.0;
 					for (int w = 0; w < pwaves; ++w)
-						t += s_part[threadIdx.x * pwaves + w];
-					s_c[threadIdx.x] = t;
-					s_H[j * (pm_max + 1) + threadIdx.x] = (pass == 0 ? 0.0 : s_H[j * (pm_max + 1) + threadIdx.x]) + t;
+						tt += s_part[threadIdx.x * pwaves + w];
+					s_c[threadIdx.x] = tt;
+					s_H[j * (pm_max + 1) + threadIdx.x] = (pass == 0 ? 0.0 : s_H[j * (pm_max + 1) + threadIdx.x]) + tt;
 				}
 				__syncthreads();
 				double w2 = 0.0;
@@ -957,7 +1167,6 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 				__threadfence_block();
 				__syncthreads();
 			}
-			// Givens rotations on column j (thread 0)
 			if (threadIdx.x == 0)
 			{
 				double* col = s_H + j * (pm_max + 1);
@@ -992,27 +1201,35 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 			if (s_scalar[1] <= 0.5 * tol || scale == 0.0 || iters >= max_iters)
 				break;
 		}
-		// y = H^-1 g, z += sum_i y_i V_i
+		// y = H^-1 g;  x += P^{-1} (sum_i y_i V_i)   (FP64)
 		if (threadIdx.x == 0)
 			for (int i = k - 1; i >= 0; --i)
 			{
-				double t = s_g[i];
+				double tt = s_g[i];
 				for (int l = i + 1; l < k; ++l)
-					t -= s_H[l * (pm_max + 1) + i] * s_c[l];
+					tt -= s_H[l * (pm_max + 1) + i] * s_c[l];
 				const double d = s_H[i * (pm_max + 1) + i];
-				s_c[i] = d != 0.0 ? t / d : 0.0;
+				s_c[i] = d != 0.0 ? tt / d : 0.0;
 			}
 		__syncthreads();
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
 		{
-			double acc = zb[u];
+			double acc = 0.0;
 			for (int i = 0; i < k; ++i)
 				acc += s_c[i] * (double)V[(int64_t)i * nU + u];
-			zb[u] = acc;
+			rr_[u] = ILU ? acc : acc * ib[u];
 		}
 		__threadfence_block();
 		__syncthreads();
+		if (ILU)
+			ilu_apply<double>(S, G, n_seg, b, ud, s_rates, rr_);
+		for (int64_t u = threadIdx.x; u < nU; u += pblock)
+			x[u] += rr_[u];
+		__threadfence_block();
+		__syncthreads();
 	}
+	for (int64_t u = threadIdx.x; u < nU; u += pblock)
+		x[u] /= ib[u];
 	if (threadIdx.x == 0)
 	{
 		out.iters[b] = iters;
@@ -1020,7 +1237,6 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, int n_seg, const 
 		out.res[b] = rel;
 	}
 }
-
 
 // ============================================================================
 // full mode: terminal kernels of every terminal SCC and the absorption
@@ -1281,13 +1497,12 @@ struct gmres_t
 // large system (few sets) is faster spread over the whole device.
 int64_t per_set_max_unknowns(int B) { return B >= 64 ? 65536 : (B >= 8 ? 16384 : 4096); }
 
-// algorithmic bytes of one per-set GMRES launch, per iteration-free estimate:
-// structure once, then per set the rhs, z (read + write) and 1/D.
-double gmres_set_bytes(const sys_t& S, int m, int tbytes)
+// algorithmic bytes of one per-set GMRES launch: the coupling structure once
+// (row pointers, neighbour index and code), then per set the right-hand side,
+// the iterate (read and written) and 1/D'.
+double gmres_set_bytes(const sys_t& S, int64_t nnz)
 {
-	(void)m;
-	(void)tbytes;
-	return (double)S.nU * 16.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
+	return (double)S.nU * 12.0 + (double)nnz * 5.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
 }
 
 // Solves A_p z = rhs for every set of S (z holds the initial guess): one
@@ -1311,34 +1526,77 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	if (nU <= per_set_max_unknowns(B))
 	{
 		const int m = std::min(restart, pm_max);
-		dbuf<double> r;
+		// coupling structure and popcount levels (shared by every set)
+		const int nb = ceil_div(nU, sblock);
+		dbuf<int32_t> cnt, ptr, mid, lvl_key, lvl;
+		cnt.ensure(nU);
+		ptr.ensure(nU + 1);
+		mid.ensure(nU);
+		lvl_key.ensure(nU);
+		lvl.ensure(nU);
+		k_nbr_count<<<nb, sblock, 0, c.stream>>>(S, cnt.get());
+		exclusive_scan_i32(c, cnt.get(), ptr.get(), nU);
+		int32_t nnz = 0;
+		ESL_HIP(hipMemcpyAsync(&nnz, ptr.get() + nU, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		dbuf<int32_t> nu;
+		dbuf<uint8_t> nr;
+		nu.ensure(std::max(nnz, 1));
+		nr.ensure(std::max(nnz, 1));
+		k_nbr_fill<<<nb, sblock, 0, c.stream>>>(S, ptr.get(), nu.get(), nr.get(), mid.get(), lvl_key.get());
+		// group unknowns by level (counting sort on the host: N + 1 buckets)
+		std::vector<int32_t> h_key;
+		read_host(c, h_key, lvl_key.get(), (size_t)nU);
+		levels_t L{};
+		L.n = S.N + 1;
+		std::vector<int32_t> h_cnt(L.n + 1, 0), h_lvl(nU);
+		for (int32_t k : h_key)
+			h_cnt[k + 1]++;
+		for (int l = 0; l < L.n; ++l)
+			h_cnt[l + 1] += h_cnt[l];
+		for (int l = 0; l <= L.n; ++l)
+			L.off[l] = h_cnt[l];
+		std::vector<int32_t> fill(h_cnt.begin(), h_cnt.end() - 1);
+		for (int64_t u = 0; u < nU; ++u)
+			h_lvl[fill[h_key[u]]++] = (int32_t)u;
+		write_dev(c, lvl.get(), h_lvl);
+		nbr_t G{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L };
+
+		dbuf<double> r, udiag;
 		dbuf<int32_t> its, cyc;
 		dbuf<double> res;
 		r.ensure((size_t)B * nU);
+		udiag.ensure((size_t)B * nU);
 		its.ensure(B);
 		cyc.ensure(B);
 		res.ensure(B);
 		pgm_out_t po{ its.get(), cyc.get(), res.get() };
 		const int nseg1 = std::max(n_seg, 1);
+		const bool ilu = opts.preconditioner != 1; // ILU(0) unless Jacobi is asked for
+		const double by = gmres_set_bytes(S, nnz);
+		auto launch = [&](auto tag) {
+			using T = decltype(tag);
+			dbuf<T> V, t;
+			V.ensure((size_t)B * (m + 1) * nU);
+			t.ensure((size_t)B * nU);
+			pgm_scratch_t sc{ r.get(), udiag.get(), t.get(), V.get() };
+			const char* name = sizeof(T) == 4 ? (ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>")
+											   : (ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>");
+			timed_launch(c, name, by, [&] {
+				if (ilu)
+					k_gmres_set<T, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs_p, z_p, sc, m, tol, max_iters,
+																	 po);
+				else
+					k_gmres_set<T, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs_p, z_p, sc, m, tol, max_iters,
+																	  po);
+			});
+			ESL_HIP(hipGetLastError());
+			ESL_HIP(hipStreamSynchronize(c.stream));
+		};
 		if (opts.precision == 1)
-		{
-			dbuf<float> V;
-			V.ensure((size_t)B * (m + 1) * nU);
-			timed_launch(c, "k_gmres_set<float>", gmres_set_bytes(S, m, 4), [&] {
-				k_gmres_set<float><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs_p, z_p, V.get(), r.get(), m, tol,
-															   max_iters, po);
-			});
-		}
+			launch(float{});
 		else
-		{
-			dbuf<double> V;
-			V.ensure((size_t)B * (m + 1) * nU);
-			timed_launch(c, "k_gmres_set<double>", gmres_set_bytes(S, m, 8), [&] {
-				k_gmres_set<double><<<B, pblock, 0, c.stream>>>(S, nseg1, rhs_p, z_p, V.get(), r.get(), m, tol,
-																max_iters, po);
-			});
-		}
-		ESL_HIP(hipGetLastError());
+			launch(double{});
 		std::vector<int32_t> h_its, h_cyc;
 		std::vector<double> h_res;
 		read_host(c, h_its, its.get(), B);

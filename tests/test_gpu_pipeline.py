@@ -192,3 +192,22 @@ def test_solution_export_vs_oracle(ctx, name):
     np.testing.assert_array_equal(L.indices, ref.nonterm_indices)
     np.testing.assert_allclose(L.data, ref.nonterm_data, rtol=1e-10, atol=1e-13)
     assert l1(s.final_state, ref.final_state) <= 1e-12
+
+
+@pytest.mark.parametrize("pre", [1, 2])
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("name", ["solver.kras", "solver.mammal", "symbolic_with_refactor.cohen"])
+def test_preconditioners_and_precisions(ctx, name, pre, prec):
+    # Jacobi and ILU(0), FP64 and FP32-with-FP64-refinement: same answer
+    c = G[name]
+    tol = 1e-12
+    m = model_builder().construct_model(model_path(c["model"]))
+    r = product_rates(m, c, "rates")
+    t, g = build(ctx, m, r.rates)
+    st = initial_state(m.nodes, c.get("fixed", []), [bool(v) for v in c.get("vals", [])], c.get("p", 1.0))
+    s = solver(ctx, t, g, r, st, preconditioner=pre, precision=prec, tol=tol)
+    s.solve()
+    nz, v = nonzero(s.final_state)
+    assert nz.tolist() == c["nonzero_indices"]
+    np.testing.assert_allclose(v, c["nonzero_data"], rtol=0, atol=c["tol"])
+    assert s.report["residual"] <= tol
