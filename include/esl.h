@@ -136,6 +136,7 @@ typedef struct
 	int restart;		 /* GMRES restart length (default 32) */
 	int full;			 /* 1: also build solution_term / solution_nonterm (esl_solution_export) */
 	int preconditioner;	 /* 0 = default (ILU(0)), 1 = Jacobi, 2 = ILU(0) (level-scheduled) */
+	int graph;			 /* 1: capture the numeric sequence of a plan in a HIP graph and replay it */
 	const double* warm_pi; /* optional warm start of the terminal kernels: predicted final-vector values
 							* on the terminal states in canonical order (reordered_vertices[0, n_t)),
 							* [n_sets][n_t]; host memory for esl_solve, device memory otherwise */
@@ -185,6 +186,27 @@ esl_status esl_solution_export(esl_ctx* ctx, const esl_graph* g, int32_t* term_i
 esl_status esl_solve_batch_device(esl_ctx* ctx, const esl_graph* g, int n_sets, const double* d_rates,
 								  const double* d_x0, double* d_final, const esl_solve_opts* opts,
 								  esl_solve_report* report);
+
+/* ---- plans: the rate-independent part of a solve, reused across rate sets ---
+ * A plan fixes one graph and one initial state (copied) and holds everything a
+ * solve needs that does not depend on the rates: reachability, which terminal
+ * SCCs each state can reach, the unknowns of the linear systems and their
+ * coupling structure and popcount levels.  The reference's counterpart is the
+ * symbolic reuse of a .symb file (solver(ctx, persisted, rates, state),
+ * cuda/src/solver.cu:25-51; refactor, linear_system_solve.cu:327-490).
+ * Every rate set solved with a plan must have the table's zero pattern. */
+typedef struct esl_plan esl_plan;
+esl_status esl_plan_create(esl_ctx* ctx, const esl_graph* g, const double* d_x0, esl_plan** out);
+void esl_plan_destroy(esl_plan* p);
+/* info[0] unknowns per set, [1] terminal-kernel unknowns, [2] ambiguous transient
+ * states, [3] reachable terminal SCCs, [4] terminal SCCs with >= 2 states,
+ * [5] coupling nonzeros, [6] n_t, [7] reserved */
+esl_status esl_plan_query(const esl_plan* p, int64_t info[8]);
+/* n_sets rate sets: d_rates [n_sets][2N], d_final [n_sets][2^N] (device).  With
+ * opts.graph = 1 the numeric sequence is captured in a HIP graph on the first
+ * call and replayed while n_sets, the pointers and the options stay the same. */
+esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+						  const esl_solve_opts* opts, esl_solve_report* report);
 
 /* ---- device memory helpers (for callers without their own allocator) ------- */
 esl_status esl_device_alloc(esl_ctx* ctx, size_t bytes, void** out);

@@ -12,6 +12,7 @@ from .api import (  # noqa: F401
     model_builder,
     model_t,
     solve_opts,
+    solve_plan,
     solver,
     sparse_cs_matrix,
     transition_graph,

@@ -30,6 +30,7 @@ class SolveOpts(C.Structure):
         ("restart", C.c_int),
         ("full", C.c_int),
         ("preconditioner", C.c_int),
+        ("graph", C.c_int),
         ("warm_pi", C.POINTER(C.c_double)),
     ]
 
@@ -96,6 +97,10 @@ SIGNATURES = [
     ("esl_memcpy_d2h", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_graph_stage_ms", C.c_int, [_VP, _P(C.c_double)]),
     ("esl_profile", C.c_int, [_VP, C.c_int]),
+    ("esl_plan_create", C.c_int, [_VP, _VP, _VP, _P(_VP)]),
+    ("esl_plan_destroy", None, [_VP]),
+    ("esl_plan_query", C.c_int, [_VP, _P(C.c_int64)]),
+    ("esl_plan_solve", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
     ("esl_mlp_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _P(_VP)]),
     ("esl_mlp_destroy", None, [_VP]),
     ("esl_mlp_num_params", C.c_int64, [_VP]),

@@ -352,3 +352,44 @@ class solver:
         else:
             print(text, end="")
         return text
+
+
+class solve_plan:
+    """Rate-independent part of a solve (one graph, one initial state), reused
+    for any number of rate sets of the same zero pattern (include/esl.h
+    esl_plan_*).  Device buffers come from paraexastolog_amd.surrogate.DeviceArray."""
+
+    def __init__(self, context: cu_context, table: transition_table, d_x0):
+        self.context = context
+        self.table = table
+        h = C.c_void_p()
+        check(lib().esl_plan_create(context.handle, table.handle, d_x0, C.byref(h)), context.handle)
+        self.handle = h
+
+    def info(self):
+        a = np.zeros(8, dtype=np.int64)
+        check(lib().esl_plan_query(self.handle, a.ctypes.data_as(C.POINTER(C.c_int64))))
+        keys = ["unknowns", "terminal_unknowns", "ambiguous", "terminals_reached", "terminal_segments", "nnz", "n_t"]
+        return dict(zip(keys, a.tolist()))
+
+    def solve(self, n_sets, d_rates, d_final, **opts):
+        o = solve_opts(**{k: v for k, v in opts.items() if k not in ("graph", "warm")})
+        o.graph = int(opts.get("graph", 0))
+        warm = opts.get("warm")
+        if warm is not None:
+            o.warm_pi = C.cast(warm, C.POINTER(C.c_double))
+        rep = SolveReport()
+        check(lib().esl_plan_solve(self.context.handle, self.handle, n_sets, d_rates, d_final, C.byref(o),
+                                   C.byref(rep)), self.context.handle)
+        return rep.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().esl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

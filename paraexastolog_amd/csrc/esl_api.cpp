@@ -28,6 +28,18 @@ struct esl_graph
 	esl::graph_t g;
 };
 
+struct esl_plan
+{
+	esl_ctx* ctx = nullptr;
+	const esl_graph* graph = nullptr;
+	void* p = nullptr;
+	~esl_plan()
+	{
+		if (p)
+			esl::plan_destroy(p);
+	}
+};
+
 namespace {
 
 thread_local std::string tl_error;
@@ -384,6 +396,7 @@ void esl_solve_opts_default(esl_solve_opts* o)
 	o->restart = 32;
 	o->full = 0;
 	o->preconditioner = 0;
+	o->graph = 0;
 	o->warm_pi = nullptr;
 }
 
@@ -522,6 +535,52 @@ esl_status esl_profile_stat(esl_ctx* ctx, int index, char* name_out, int name_ca
 			*total_ms = e.second.ms;
 		if (total_bytes)
 			*total_bytes = e.second.bytes;
+	});
+}
+
+// ---------------------------------------------------------------- plans
+
+esl_status esl_plan_create(esl_ctx* ctx, const esl_graph* g, const double* d_x0, esl_plan** out)
+{
+	return guarded(ctx, [&] {
+		require(ctx && g && d_x0 && out, ESL_E_ARG, "null argument");
+		set_device(ctx);
+		auto pl = std::make_unique<esl_plan>();
+		pl->ctx = ctx;
+		pl->graph = g;
+		pl->p = esl::plan_create(ctx->c, const_cast<esl::graph_t&>(g->g), d_x0);
+		*out = pl.release();
+	});
+}
+
+void esl_plan_destroy(esl_plan* p) { delete p; }
+
+esl_status esl_plan_query(const esl_plan* p, int64_t info[8])
+{
+	return guarded(nullptr, [&] {
+		require(p && info, ESL_E_ARG, "null argument");
+		esl::plan_info(p->p, info);
+	});
+}
+
+esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+						  const esl_solve_opts* opts, esl_solve_report* report)
+{
+	return guarded(ctx, [&] {
+		require(ctx && p && d_rates && d_final, ESL_E_ARG, "null argument");
+		require(p->ctx == ctx, ESL_E_ARG, "plan belongs to another context");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		set_device(ctx);
+		esl_solve_opts o;
+		esl_solve_opts_default(&o);
+		if (opts)
+			o = *opts;
+		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		require(!o.full, ESL_E_ARG, "opts.full is not available on plans; use esl_solve");
+		esl_solve_report rep{};
+		esl::plan_solve(ctx->c, p->p, n_sets, d_rates, d_final, o, rep);
+		if (report)
+			*report = rep;
 	});
 }
 

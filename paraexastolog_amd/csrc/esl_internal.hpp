@@ -92,9 +92,20 @@ struct prof_t
 	std::vector<hipEvent_t> pool;
 	size_t used = 0;
 	std::vector<std::pair<std::string, kstat_t>> stats;
+	// while a HIP graph is being captured: its event pairs go to the graph's owner
+	std::vector<hipEvent_t>* capture_events = nullptr;
+	std::vector<pending_t>* capture_pending = nullptr;
 
 	hipEvent_t next_event()
 	{
+		if (capture_events)
+		{
+			hipEvent_t e;
+			if (hipEventCreate(&e) != hipSuccess)
+				throw error(ESL_E_DEVICE, "hipEventCreate failed");
+			capture_events->push_back(e);
+			return e;
+		}
 		if (used == pool.size())
 		{
 			hipEvent_t e;
@@ -112,10 +123,10 @@ struct prof_t
 		stats.emplace_back(name, kstat_t{});
 		return stats.back().second;
 	}
-	// resolves the recorded event pairs (the stream must be idle)
-	void flush()
+	// books the event pairs of one completed launch sequence (the stream must be idle)
+	void book(const std::vector<pending_t>& list)
 	{
-		for (auto& p : pending)
+		for (auto& p : list)
 		{
 			float ms = 0;
 			if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
@@ -126,6 +137,10 @@ struct prof_t
 				k.bytes += p.bytes;
 			}
 		}
+	}
+	void flush()
+	{
+		book(pending);
 		pending.clear();
 		used = 0;
 	}
@@ -163,6 +178,11 @@ inline void timed_launch(ctx_t& c, const char* name, double bytes, F&& launch)
 	ESL_HIP(hipEventRecord(a, c.stream));
 	launch();
 	ESL_HIP(hipEventRecord(b, c.stream));
+	if (c.prof.capture_pending)
+	{
+		c.prof.capture_pending->push_back({ name, a, b, bytes });
+		return;
+	}
 	c.prof.pending.push_back({ name, a, b, bytes });
 	if (c.prof.pending.size() > 4096)
 	{
@@ -224,6 +244,12 @@ struct solve_args_t
 	esl_solve_opts opts;
 };
 void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep);
+// rate-independent solve plans (solve.hip)
+void* plan_create(ctx_t& c, graph_t& g, const double* d_x0);
+void plan_destroy(void* p);
+void plan_solve(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
+				esl_solve_report& rep);
+void plan_info(const void* p, int64_t info[8]);
 // the implementation object behind an esl_ctx handle (defined in esl_api.cpp)
 ctx_t& ctx_impl(esl_ctx* ctx);
 // solution_term / solution_nonterm of one rate set into g.sol (reference layout)

@@ -248,14 +248,12 @@ __global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32
 __global__ __launch_bounds__(gblock) void k_root_keys(int64_t n, const int32_t* __restrict__ scc,
 													  const int32_t* __restrict__ h, int32_t H,
 													  const int32_t* __restrict__ root_pos,
-													  uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
-													  int32_t* __restrict__ size)
+													  uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
 {
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	if (v >= n)
 		return;
 	const int32_t r = scc[v];
-	atomicAdd(&size[r], 1);
 	if (r != (int32_t)v)
 		return;
 	const int32_t hv = h[v];
@@ -275,15 +273,26 @@ __global__ __launch_bounds__(gblock) void k_is_root(int64_t n, const int32_t* __
 }
 
 __global__ __launch_bounds__(gblock) void k_rank_roots(int64_t S, const int32_t* __restrict__ sorted_roots,
-													   const int32_t* __restrict__ size, int32_t* __restrict__ rank,
-													   int32_t* __restrict__ sorted_size)
+													   int32_t* __restrict__ rank)
 {
 	const int64_t k = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	if (k >= S)
 		return;
-	const int32_t r = sorted_roots[k];
-	rank[r] = (int32_t)k; // rank is indexed by root vertex here
-	sorted_size[k] = size[r];
+	rank[sorted_roots[k]] = (int32_t)k; // rank is indexed by root vertex here
+}
+
+// sccs_offsets from the vertices sorted by SCC rank: offsets[k] = first position of rank k
+__global__ __launch_bounds__(gblock) void k_seg_bounds(int64_t n, int64_t S, const uint32_t* __restrict__ sorted_keys,
+													   int32_t* __restrict__ offsets)
+{
+	const int64_t p = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (p >= n)
+		return;
+	const uint32_t k = sorted_keys[p];
+	if (p == 0 || sorted_keys[p - 1] != k)
+		offsets[k] = (int32_t)p;
+	if (p == n - 1)
+		offsets[S] = (int32_t)n;
 }
 
 __global__ __launch_bounds__(gblock) void k_vertex_keys(int64_t n, const int32_t* __restrict__ scc,
@@ -417,15 +426,13 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 
 	// sort roots by key
 	dbuf<uint64_t> rkeys, rkeys_out;
-	dbuf<int32_t> rvals, rvals_out, size;
+	dbuf<int32_t> rvals, rvals_out;
 	rkeys.ensure(S);
 	rkeys_out.ensure(S);
 	rvals.ensure(S);
 	rvals_out.ensure(S);
-	size.ensure(n);
-	ESL_HIP(hipMemsetAsync(size.get(), 0, n * sizeof(int32_t), c.stream));
 	k_root_keys<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), g.H, root_pos.get(), rkeys.get(),
-											 rvals.get(), size.get());
+											 rvals.get());
 	ESL_HIP(hipGetLastError());
 	size_t tmp_bytes = 0;
 	ESL_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, rkeys.get(), rkeys_out.get(), rvals.get(),
@@ -434,13 +441,9 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	tmp.ensure(tmp_bytes);
 	ESL_HIP(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, rkeys.get(), rkeys_out.get(), rvals.get(),
 									  rvals_out.get(), (size_t)S, 0, 64, c.stream));
-	dbuf<int32_t> root_rank, sorted_size;
+	dbuf<int32_t> root_rank;
 	root_rank.ensure(n);
-	sorted_size.ensure(S + 1);
-	k_rank_roots<<<ceil_div(S, gblock), gblock, 0, c.stream>>>(S, rvals_out.get(), size.get(), root_rank.get(),
-																sorted_size.get());
-	g.offsets.ensure(S + 1);
-	exclusive_scan_i32(c, sorted_size.get(), g.offsets.get(), S);
+	k_rank_roots<<<ceil_div(S, gblock), gblock, 0, c.stream>>>(S, rvals_out.get(), root_rank.get());
 
 	// stable sort of the vertices by their SCC's rank
 	dbuf<uint32_t> vkeys, vkeys_out;
@@ -463,6 +466,9 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	tmp.ensure(tmp_bytes);
 	ESL_HIP(rocprim::radix_sort_pairs(tmp.get(), tmp_bytes, vkeys.get(), vkeys_out.get(), vvals.get(),
 									  g.reordered.get(), (size_t)n, 0, end_bit, c.stream));
+	g.offsets.ensure(S + 1);
+	k_seg_bounds<<<nb, gblock, 0, c.stream>>>(n, S, vkeys_out.get(), g.offsets.get());
+	ESL_HIP(hipGetLastError());
 	ESL_HIP(hipEventRecord(c.ev[3], c.stream));
 	ESL_HIP(hipEventSynchronize(c.ev[3]));
 	float ms = 0;

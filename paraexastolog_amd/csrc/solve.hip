@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 
 #include "common.hip.hpp"
 #include "esl_internal.hpp"
@@ -613,6 +614,13 @@ __global__ __launch_bounds__(sblock) void k_cast_scale(int64_t nU, const double*
 		return;
 	const double s = scale[b];
 	out[(int64_t)b * nU + u] = (T)(s > 0.0 ? r[(int64_t)b * nU + u] / s : 0.0);
+}
+
+__global__ __launch_bounds__(sblock) void k_fill_value(int64_t count, double v, double* __restrict__ p)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		p[i] = v;
 }
 
 template <typename T>
@@ -1514,6 +1522,205 @@ struct krylov_result_t
 	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
 };
 
+// coupling structure of the per-set solver (see k_nbr_fill)
+struct nbr_store_t
+{
+	bool ready = false;
+	dbuf<int32_t> ptr, mid, nu, lvl;
+	dbuf<uint8_t> nr;
+	levels_t L{};
+	int64_t nnz = 0;
+	nbr_t view() const { return nbr_t{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L }; }
+};
+
+void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
+{
+	const int64_t nU = S.nU;
+	const int nb = ceil_div(nU, sblock);
+	dbuf<int32_t> cnt, lvl_key;
+	cnt.ensure(nU);
+	lvl_key.ensure(nU);
+	st.ptr.ensure(nU + 1);
+	st.mid.ensure(nU);
+	st.lvl.ensure(nU);
+	k_nbr_count<<<nb, sblock, 0, c.stream>>>(S, cnt.get());
+	exclusive_scan_i32(c, cnt.get(), st.ptr.get(), nU);
+	int32_t nnz = 0;
+	ESL_HIP(hipMemcpyAsync(&nnz, st.ptr.get() + nU, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	st.nnz = nnz;
+	st.nu.ensure(std::max(nnz, 1));
+	st.nr.ensure(std::max(nnz, 1));
+	k_nbr_fill<<<nb, sblock, 0, c.stream>>>(S, st.ptr.get(), st.nu.get(), st.nr.get(), st.mid.get(), lvl_key.get());
+	// unknowns grouped by popcount level (counting sort on the host: N + 1 buckets)
+	std::vector<int32_t> h_key;
+	read_host(c, h_key, lvl_key.get(), (size_t)nU);
+	st.L = levels_t{};
+	st.L.n = S.N + 1;
+	std::vector<int32_t> h_cnt(st.L.n + 1, 0), h_lvl(nU);
+	for (int32_t k : h_key)
+		h_cnt[k + 1]++;
+	for (int l = 0; l < st.L.n; ++l)
+		h_cnt[l + 1] += h_cnt[l];
+	for (int l = 0; l <= st.L.n; ++l)
+		st.L.off[l] = h_cnt[l];
+	std::vector<int32_t> fill(h_cnt.begin(), h_cnt.end() - 1);
+	for (int64_t u = 0; u < nU; ++u)
+		h_lvl[fill[h_key[u]]++] = (int32_t)u;
+	write_dev(c, st.lvl.get(), h_lvl);
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	st.ready = true;
+}
+
+
+// scratch of the per-set GMRES kernel, grown on demand and reused
+struct per_set_scratch_t
+{
+	dbuf<double> r, udiag, res;
+	dbuf<int32_t> its, cyc;
+	dbuf<float> Vf, tf;
+	dbuf<double> Vd, td;
+	void reserve(int B, int64_t nU, int m, int precision)
+	{
+		nU = std::max<int64_t>(nU, 1);
+		r.ensure((size_t)B * nU);
+		udiag.ensure((size_t)B * nU);
+		res.ensure(B);
+		its.ensure(B);
+		cyc.ensure(B);
+		if (precision == 1)
+		{
+			Vf.ensure((size_t)B * (m + 1) * nU);
+			tf.ensure((size_t)B * nU);
+		}
+		else
+		{
+			Vd.ensure((size_t)B * (m + 1) * nU);
+			td.ensure((size_t)B * nU);
+		}
+	}
+	krylov_result_t collect(ctx_t& c, int B, double tol, bool any)
+	{
+		krylov_result_t kr;
+		if (!any)
+			return kr;
+		std::vector<int32_t> h_its, h_cyc;
+		std::vector<double> h_res;
+		read_host(c, h_its, its.get(), B);
+		read_host(c, h_cyc, cyc.get(), B);
+		read_host(c, h_res, res.get(), B);
+		for (int b = 0; b < B; ++b)
+		{
+			kr.resid = std::max(kr.resid, std::isnan(h_res[b]) ? 1e300 : h_res[b]);
+			kr.iters = std::max(kr.iters, h_its[b]);
+			kr.cycles = std::max(kr.cycles, h_cyc[b]);
+			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
+		}
+		if (!(kr.resid <= tol))
+			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+										   std::to_string(kr.resid) + " after " + std::to_string(kr.iters) +
+										   " iterations");
+		return kr;
+	}
+};
+
+// one launch: GMRES of every set, one workgroup per set (no host synchronisation)
+void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs, double* z,
+					const esl_solve_opts& o, per_set_scratch_t& sc)
+{
+	const int B = S.B;
+	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
+	const int nseg1 = std::max(n_seg, 1);
+	const bool ilu = o.preconditioner != 1; // ILU(0) unless Jacobi is asked for
+	const nbr_t G = nb.view();
+	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get() };
+	const double by = gmres_set_bytes(S, nb.nnz);
+	if (o.precision == 1)
+	{
+		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.tf.get(), sc.Vf.get() };
+		timed_launch(c, ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>", by, [&] {
+			if (ilu)
+				k_gmres_set<float, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
+																	 po);
+			else
+				k_gmres_set<float, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																	  o.max_iters, po);
+		});
+	}
+	else
+	{
+		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.td.get(), sc.Vd.get() };
+		timed_launch(c, ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>", by, [&] {
+			if (ilu)
+				k_gmres_set<double, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
+																	  po);
+			else
+				k_gmres_set<double, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																	   o.max_iters, po);
+		});
+	}
+	ESL_HIP(hipGetLastError());
+}
+
+// everything a solve needs that does not depend on the rates
+struct plan_t
+{
+	graph_t* g = nullptr;
+	dbuf<double> x0;
+	dbuf<uint8_t> reach;
+	dbuf<uint64_t> tmask;
+	dbuf<int32_t> amb, pos, ulist, ucls, d_jmap, d_segmap, d_seg_off, d_tpos;
+	std::vector<int32_t> jmap, segmap, seg_off, seg_src;
+	int Tr = 0, n_seg = 0;
+	bool fallback = false;
+	int64_t nU = 0, nUt = 0, nA = 0;
+	dbuf<double> y0;
+	nbr_store_t nbr;
+	// per-run buffers (grown to the largest batch seen)
+	dbuf<double> p0, p1, invd, rhs, z, y, segsum, fpart;
+	dbuf<int32_t> anchor;
+	per_set_scratch_t scr;
+	int anchor_sweeps = 32;
+	double ms[3] = { 0, 0, 0 };
+	// captured numeric sequence and the arguments it was captured with
+	struct gkey_t
+	{
+		int B = 0;
+		const double* rates = nullptr;
+		double* fin = nullptr;
+		const double* warm = nullptr;
+		int precision = 0, preconditioner = 0, m = 0;
+		double tol = 0;
+		int max_iters = 0;
+		bool operator==(const gkey_t& o) const
+		{
+			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
+				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && max_iters == o.max_iters;
+		}
+	};
+	gkey_t gkey{};
+	bool gprof = false; // captured with profiling events
+	hipGraphExec_t gexec = nullptr;
+	std::vector<hipEvent_t> gev;			   // events recorded by the graph
+	std::vector<prof_t::pending_t> gpend;	   // kernels timed inside the graph
+	plan_t() = default;
+	plan_t(const plan_t&) = delete;
+	plan_t& operator=(const plan_t&) = delete;
+	void drop_graph()
+	{
+		if (gexec)
+			(void)hipGraphExecDestroy(gexec);
+		gexec = nullptr;
+		for (auto e : gev)
+			(void)hipEventDestroy(e);
+		gev.clear();
+		gpend.clear();
+	}
+	~plan_t() { drop_graph(); }
+	sys_t sys(int B, const double* rates) const;
+};
+
+
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
 							 const esl_solve_opts& opts)
 {
@@ -1526,93 +1733,17 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	if (nU <= per_set_max_unknowns(B))
 	{
 		const int m = std::min(restart, pm_max);
-		// coupling structure and popcount levels (shared by every set)
-		const int nb = ceil_div(nU, sblock);
-		dbuf<int32_t> cnt, ptr, mid, lvl_key, lvl;
-		cnt.ensure(nU);
-		ptr.ensure(nU + 1);
-		mid.ensure(nU);
-		lvl_key.ensure(nU);
-		lvl.ensure(nU);
-		k_nbr_count<<<nb, sblock, 0, c.stream>>>(S, cnt.get());
-		exclusive_scan_i32(c, cnt.get(), ptr.get(), nU);
-		int32_t nnz = 0;
-		ESL_HIP(hipMemcpyAsync(&nnz, ptr.get() + nU, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		nbr_store_t nb;
+		build_nbr(c, S, nb);
+		per_set_scratch_t sc;
+		sc.reserve(B, nU, m, opts.precision);
+		esl_solve_opts o = opts;
+		o.tol = tol;
+		o.max_iters = max_iters;
+		o.restart = m;
+		launch_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc);
 		ESL_HIP(hipStreamSynchronize(c.stream));
-		dbuf<int32_t> nu;
-		dbuf<uint8_t> nr;
-		nu.ensure(std::max(nnz, 1));
-		nr.ensure(std::max(nnz, 1));
-		k_nbr_fill<<<nb, sblock, 0, c.stream>>>(S, ptr.get(), nu.get(), nr.get(), mid.get(), lvl_key.get());
-		// group unknowns by level (counting sort on the host: N + 1 buckets)
-		std::vector<int32_t> h_key;
-		read_host(c, h_key, lvl_key.get(), (size_t)nU);
-		levels_t L{};
-		L.n = S.N + 1;
-		std::vector<int32_t> h_cnt(L.n + 1, 0), h_lvl(nU);
-		for (int32_t k : h_key)
-			h_cnt[k + 1]++;
-		for (int l = 0; l < L.n; ++l)
-			h_cnt[l + 1] += h_cnt[l];
-		for (int l = 0; l <= L.n; ++l)
-			L.off[l] = h_cnt[l];
-		std::vector<int32_t> fill(h_cnt.begin(), h_cnt.end() - 1);
-		for (int64_t u = 0; u < nU; ++u)
-			h_lvl[fill[h_key[u]]++] = (int32_t)u;
-		write_dev(c, lvl.get(), h_lvl);
-		nbr_t G{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L };
-
-		dbuf<double> r, udiag;
-		dbuf<int32_t> its, cyc;
-		dbuf<double> res;
-		r.ensure((size_t)B * nU);
-		udiag.ensure((size_t)B * nU);
-		its.ensure(B);
-		cyc.ensure(B);
-		res.ensure(B);
-		pgm_out_t po{ its.get(), cyc.get(), res.get() };
-		const int nseg1 = std::max(n_seg, 1);
-		const bool ilu = opts.preconditioner != 1; // ILU(0) unless Jacobi is asked for
-		const double by = gmres_set_bytes(S, nnz);
-		auto launch = [&](auto tag) {
-			using T = decltype(tag);
-			dbuf<T> V, t;
-			V.ensure((size_t)B * (m + 1) * nU);
-			t.ensure((size_t)B * nU);
-			pgm_scratch_t sc{ r.get(), udiag.get(), t.get(), V.get() };
-			const char* name = sizeof(T) == 4 ? (ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>")
-											   : (ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>");
-			timed_launch(c, name, by, [&] {
-				if (ilu)
-					k_gmres_set<T, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs_p, z_p, sc, m, tol, max_iters,
-																	 po);
-				else
-					k_gmres_set<T, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs_p, z_p, sc, m, tol, max_iters,
-																	  po);
-			});
-			ESL_HIP(hipGetLastError());
-			ESL_HIP(hipStreamSynchronize(c.stream));
-		};
-		if (opts.precision == 1)
-			launch(float{});
-		else
-			launch(double{});
-		std::vector<int32_t> h_its, h_cyc;
-		std::vector<double> h_res;
-		read_host(c, h_its, its.get(), B);
-		read_host(c, h_cyc, cyc.get(), B);
-		read_host(c, h_res, res.get(), B);
-		for (int b = 0; b < B; ++b)
-		{
-			kr.resid = std::max(kr.resid, h_res[b]);
-			kr.iters = std::max(kr.iters, h_its[b]);
-			kr.cycles = std::max(kr.cycles, h_cyc[b]);
-			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
-		}
-		if (!(kr.resid <= tol))
-			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-										   std::to_string(kr.resid) + " after " + std::to_string(kr.iters) +
-										   " iterations");
+		kr = sc.collect(c, B, tol, true);
 	}
 	else if (opts.precision == 1)
 	{
@@ -1637,253 +1768,400 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 
 } // namespace
 
-void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep)
+// ---- plan: the rate-independent part of a solve -----------------------------
+
+void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 {
 	if (!g.reorganized)
 		throw error(ESL_E_STATE, "esl_graph_reorganize must run before a solve");
 	const int N = g.N;
 	const int64_t n = g.n;
-	const int B = a.n_sets;
 	const int64_t T = g.T;
 	const int64_t n_t = g.n_t;
 	const int nb = ceil_div(n, sblock);
 	flags_t fl(c);
 	timer_t tm_(c);
+	P.g = &g;
+	P.x0.ensure(n);
+	ESL_HIP(hipMemcpyAsync(P.x0.get(), d_x0, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
 
 	// ---- reachability from supp(x0)
-	dbuf<uint8_t> reach;
-	reach.ensure(n);
-	k_reach_init<<<nb, sblock, 0, c.stream>>>(n, a.d_x0, reach.get());
+	P.reach.ensure(n);
+	k_reach_init<<<nb, sblock, 0, c.stream>>>(n, P.x0.get(), P.reach.get());
 	for (int p = 0;; ++p)
 	{
 		fl.clear(0);
-		k_reach<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), reach.get(), fl.dev(0));
+		k_reach<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), P.reach.get(), fl.dev(0));
 		if (!fl.read(0))
 			break;
 		if (p > (1 << 24))
 			throw error(ESL_E_DEVICE, "reachability did not converge");
 	}
-	rep.ms_stage[0] = tm_.lap();
+	P.ms[0] = tm_.lap();
 
 	// ---- reachable terminal SCCs, compact index j, segments of size >= 2
 	dbuf<int32_t> reached;
-	reached.ensure(T);
+	reached.ensure(std::max<int64_t>(T, 1));
 	ESL_HIP(hipMemsetAsync(reached.get(), 0, T * sizeof(int32_t), c.stream));
 	if (n_t)
 		k_term_reached<<<ceil_div(n_t, sblock), sblock, 0, c.stream>>>(n_t, g.reordered.get(), g.rank.get(),
-																		 reach.get(), reached.get());
+																		 P.reach.get(), reached.get());
 	std::vector<int32_t> h_reached;
 	read_host(c, h_reached, reached.get(), (size_t)T);
-	std::vector<int32_t> jmap(T, -1), segmap(T, -1), seg_off(1, 0), seg_src;
-	int Tr = 0, n_seg = 0;
+	P.jmap.assign(T, -1);
+	P.segmap.assign(T, -1);
+	P.seg_off.assign(1, 0);
+	P.seg_src.clear();
+	P.Tr = 0;
+	P.n_seg = 0;
 	for (int64_t k = 0; k < T; ++k)
 	{
 		if (!h_reached[k])
 			continue;
-		jmap[k] = Tr++;
+		P.jmap[k] = P.Tr++;
 		const int32_t sz = g.h_offsets[k + 1] - g.h_offsets[k];
 		if (sz >= 2)
 		{
-			segmap[k] = n_seg++;
-			seg_off.push_back(seg_off.back() + sz);
-			seg_src.push_back(g.h_offsets[k]);
+			P.segmap[k] = P.n_seg++;
+			P.seg_off.push_back(P.seg_off.back() + sz);
+			P.seg_src.push_back(g.h_offsets[k]);
 		}
 	}
-	if (Tr > 4096)
+	if (P.Tr > 4096)
 		throw error(ESL_E_ARG, "more than 4096 terminal SCCs are reachable from the initial states");
-	const bool fallback = Tr > max_rhs_terms;
-	rep.terminals_reached = Tr;
-	dbuf<int32_t> d_jmap, d_segmap;
-	d_jmap.ensure(T);
-	d_segmap.ensure(T);
-	write_dev(c, d_jmap.get(), jmap);
-	write_dev(c, d_segmap.get(), segmap);
+	P.fallback = P.Tr > max_rhs_terms;
+	P.d_jmap.ensure(std::max<int64_t>(T, 1));
+	P.d_segmap.ensure(std::max<int64_t>(T, 1));
+	write_dev(c, P.d_jmap.get(), P.jmap);
+	write_dev(c, P.d_segmap.get(), P.segmap);
 
 	// ---- terminal masks and ambiguous set
-	dbuf<uint64_t> tmask;
-	tmask.ensure(n);
-	k_tm_init<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, d_jmap.get(), reach.get(), tmask.get());
-	if (!fallback && Tr > 1)
+	P.tmask.ensure(n);
+	k_tm_init<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, P.d_jmap.get(), P.reach.get(), P.tmask.get());
+	if (!P.fallback && P.Tr > 1)
 		for (int p = 0;; ++p)
 		{
 			fl.clear(0);
-			k_tm<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), g.rank.get(), T, reach.get(), tmask.get(),
+			k_tm<<<nb, sblock, 0, c.stream>>>(N, n, g.mask.get(), g.rank.get(), T, P.reach.get(), P.tmask.get(),
 											  fl.dev(0));
 			if (!fl.read(0))
 				break;
 			if (p > (1 << 24))
 				throw error(ESL_E_DEVICE, "terminal-mask propagation did not converge");
 		}
-	dbuf<int32_t> amb, fpos;
-	amb.ensure(n);
+	P.amb.ensure(n);
+	dbuf<int32_t> fpos;
 	fpos.ensure(n + 1);
-	if (Tr > 1)
-		k_amb_flag<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, reach.get(), tmask.get(), fallback ? 1 : 0,
-												amb.get());
+	if (P.Tr > 1)
+		k_amb_flag<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, P.reach.get(), P.tmask.get(), P.fallback ? 1 : 0,
+												P.amb.get());
 	else
-		ESL_HIP(hipMemsetAsync(amb.get(), 0, n * sizeof(int32_t), c.stream));
-	exclusive_scan_i32(c, amb.get(), fpos.get(), n);
+		ESL_HIP(hipMemsetAsync(P.amb.get(), 0, n * sizeof(int32_t), c.stream));
+	exclusive_scan_i32(c, P.amb.get(), fpos.get(), n);
 	int32_t nA = 0;
 	ESL_HIP(hipMemcpyAsync(&nA, fpos.get() + n, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
 	ESL_HIP(hipStreamSynchronize(c.stream));
-	rep.ms_stage[1] = tm_.lap();
+	P.nA = nA;
+	P.ms[1] = tm_.lap();
 
 	// ---- unknowns: terminal segments (canonical order) then ambiguous states
-	const int64_t nUt = seg_off.back();
-	const int64_t nU = nUt + nA;
-	rep.unknowns = nU;
-	dbuf<int32_t> pos, ulist, ucls, d_seg_off, d_tpos;
-	pos.ensure(n);
-	ulist.ensure(nU + 1);
-	ucls.ensure(nU + 1);
-	d_seg_off.ensure(seg_off.size());
-	ESL_HIP(hipMemsetAsync(pos.get(), 0xff, n * sizeof(int32_t), c.stream));
-	write_dev(c, d_seg_off.get(), seg_off);
-	if (nUt)
+	P.nUt = P.seg_off.back();
+	P.nU = P.nUt + nA;
+	P.pos.ensure(n);
+	P.ulist.ensure(P.nU + 1);
+	P.ucls.ensure(P.nU + 1);
+	P.d_seg_off.ensure(P.seg_off.size());
+	ESL_HIP(hipMemsetAsync(P.pos.get(), 0xff, n * sizeof(int32_t), c.stream));
+	write_dev(c, P.d_seg_off.get(), P.seg_off);
+	if (P.nUt)
 	{
-		// states of the segments, from the canonical order
 		std::vector<int32_t> h_reord;
 		read_host(c, h_reord, g.reordered.get(), (size_t)n_t);
-		std::vector<int32_t> states(nUt), seg_of(nUt), tpos(nUt);
-		for (int k = 0; k < n_seg; ++k)
-			for (int32_t q = 0; q < seg_off[k + 1] - seg_off[k]; ++q)
+		std::vector<int32_t> states(P.nUt), seg_of(P.nUt), tpos(P.nUt);
+		for (int k = 0; k < P.n_seg; ++k)
+			for (int32_t q = 0; q < P.seg_off[k + 1] - P.seg_off[k]; ++q)
 			{
-				states[seg_off[k] + q] = h_reord[seg_src[k] + q];
-				seg_of[seg_off[k] + q] = k;
-				tpos[seg_off[k] + q] = seg_src[k] + q;
+				states[P.seg_off[k] + q] = h_reord[P.seg_src[k] + q];
+				seg_of[P.seg_off[k] + q] = k;
+				tpos[P.seg_off[k] + q] = P.seg_src[k] + q;
 			}
-		d_tpos.ensure(nUt);
-		write_dev(c, d_tpos.get(), tpos);
+		P.d_tpos.ensure(P.nUt);
+		write_dev(c, P.d_tpos.get(), tpos);
 		dbuf<int32_t> d_states, d_segof;
-		d_states.ensure(nUt);
-		d_segof.ensure(nUt);
+		d_states.ensure(P.nUt);
+		d_segof.ensure(P.nUt);
 		write_dev(c, d_states.get(), states);
 		write_dev(c, d_segof.get(), seg_of);
-		k_fill_term_unknowns<<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, d_states.get(), d_segof.get(),
-																			 ulist.get(), ucls.get(), pos.get());
+		k_fill_term_unknowns<<<ceil_div(P.nUt, sblock), sblock, 0, c.stream>>>(
+			P.nUt, d_states.get(), d_segof.get(), P.ulist.get(), P.ucls.get(), P.pos.get());
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	}
 	if (nA)
-		k_fill_unknowns<<<nb, sblock, 0, c.stream>>>(n, amb.get(), fpos.get(), nUt, ulist.get(), ucls.get(),
-													 pos.get());
+		k_fill_unknowns<<<nb, sblock, 0, c.stream>>>(n, P.amb.get(), fpos.get(), P.nUt, P.ulist.get(), P.ucls.get(),
+													 P.pos.get());
 
 	// ---- x0 mass needing no solve
-	dbuf<double> y0part, y0;
-	y0part.ensure((size_t)nb * std::max(Tr, 1));
-	y0.ensure(std::max(Tr, 1));
-	if (Tr)
+	P.y0.ensure(std::max(P.Tr, 1));
+	if (P.Tr)
 	{
-		k_y0<<<nb, sblock, Tr * sizeof(double), c.stream>>>(n, g.rank.get(), T, d_jmap.get(), reach.get(),
-															 tmask.get(), amb.get(), a.d_x0, Tr, y0part.get());
-		k_reduce_parts<<<dim3(Tr, 1), wave, 0, c.stream>>>(y0part.get(), nb, Tr, y0.get(), Tr, 0);
-	}
-	ESL_HIP(hipGetLastError());
-
-	// ---- per-set system coefficients
-	sys_t S{};
-	S.N = N;
-	S.nU = nU;
-	S.nUt = nUt;
-	S.B = B;
-	S.mask = g.mask.get();
-	S.ulist = ulist.get();
-	S.ucls = ucls.get();
-	S.pos = pos.get();
-	S.rates = a.d_rates;
-	S.rate_stride = 2 * N;
-	S.transpose = 0;
-	dbuf<int32_t> anchor;
-	anchor.ensure((size_t)B * std::max(n_seg, 1));
-	dbuf<double> p0, p1;
-	const bool warm = a.opts.warm_pi != nullptr && n_seg > 0;
-	if (n_seg)
-	{
-		p0.ensure((size_t)B * nUt);
-		p1.ensure((size_t)B * nUt);
-		const dim3 gt(ceil_div(nUt, sblock), B);
-		if (warm)
-			// the prediction picks the anchors (no Jacobi sweeps)
-			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, d_tpos.get(), a.opts.warm_pi, p0.get());
-		else
-		{
-			std::vector<double> ones((size_t)B * nUt, 1.0);
-			write_dev(c, p0.get(), ones);
-			for (int it = 0; it < 32; ++it)
-			{
-				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, p0.get(), p1.get());
-				std::swap(p0, p1);
-			}
-		}
-		k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), p0.get(), n_seg, anchor.get());
+		dbuf<double> y0part;
+		y0part.ensure((size_t)nb * P.Tr);
+		k_y0<<<nb, sblock, P.Tr * sizeof(double), c.stream>>>(n, g.rank.get(), T, P.d_jmap.get(), P.reach.get(),
+															   P.tmask.get(), P.amb.get(), P.x0.get(), P.Tr,
+															   y0part.get());
+		k_reduce_parts<<<dim3(P.Tr, 1), wave, 0, c.stream>>>(y0part.get(), nb, P.Tr, P.y0.get(), P.Tr, 0);
 		ESL_HIP(hipGetLastError());
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	}
-	S.anchor = anchor.get();
-	dbuf<double> invd, rhs, z;
-	invd.ensure((size_t)B * std::max<int64_t>(nU, 1));
-	rhs.ensure((size_t)B * std::max<int64_t>(nU, 1));
-	z.ensure((size_t)B * std::max<int64_t>(nU, 1));
-	S.invd = invd.get();
-	rep.ms_stage[2] = tm_.lap();
+	// ---- coupling structure of the per-set solver
+	P.nbr.ready = false;
+	if (P.nU)
+	{
+		sys_t S = P.sys(1, nullptr);
+		build_nbr(c, S, P.nbr);
+	}
+	P.ms[2] = tm_.lap();
+}
 
-	double resid = 0.0;
-	int iters = 0, cycles = 0;
+sys_t plan_t::sys(int B, const double* rates) const
+{
+	sys_t S{};
+	S.N = g->N;
+	S.nU = nU;
+	S.nUt = nUt;
+	S.B = B;
+	S.mask = g->mask.get();
+	S.ulist = ulist.get();
+	S.ucls = ucls.get();
+	S.pos = pos.get();
+	S.rates = rates;
+	S.rate_stride = 2 * g->N;
+	S.transpose = 0;
+	S.anchor = anchor.get();
+	S.invd = invd.get();
+	return S;
+}
+
+// The numeric part for B rate sets: anchors, coefficients and start vector
+// (setup), the Krylov solves, then absorption masses and the final vector
+// (post).  No host synchronisation inside, so with the per-set solver the whole
+// sequence can be captured into a HIP graph; results are checked afterwards.
+void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const esl_solve_opts& o)
+{
+	const int64_t n_t = P.g->n_t;
+	const int64_t nU = P.nU, nUt = P.nUt;
+	const int n_seg = P.n_seg, nseg1 = std::max(n_seg, 1);
+	sys_t S = P.sys(B, d_rates);
+	const bool warm = o.warm_pi != nullptr && n_seg > 0;
+	if (n_seg)
+	{
+		const dim3 gt(ceil_div(nUt, sblock), B);
+		if (warm)
+			// the prediction picks the anchors (no Jacobi sweeps)
+			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, P.d_tpos.get(), o.warm_pi, P.p0.get());
+		else
+		{
+			k_fill_value<<<ceil_div((int64_t)B * nUt, sblock), sblock, 0, c.stream>>>((int64_t)B * nUt, 1.0,
+																					  P.p0.get());
+			for (int it = 0; it < P.anchor_sweeps; ++it)
+			{
+				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, P.p0.get(), P.p1.get());
+				std::swap(P.p0, P.p1);
+			}
+		}
+		k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.p0.get(), n_seg,
+																 P.anchor.get());
+	}
 	if (nU)
 	{
 		const dim3 gu(ceil_div(nU, sblock), B);
-		k_invd<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), invd.get());
-		k_rhs<<<gu, sblock, 0, c.stream>>>(S, std::max(n_seg, 1), a.d_x0, rhs.get());
-		k_zero<double><<<ceil_div((int64_t)B * nU, sblock), sblock, 0, c.stream>>>((int64_t)B * nU, z.get());
+		k_invd<<<gu, sblock, 0, c.stream>>>(S, nseg1, P.invd.get());
+		k_rhs<<<gu, sblock, 0, c.stream>>>(S, nseg1, P.x0.get(), P.rhs.get());
+		k_zero<double><<<ceil_div((int64_t)B * nU, sblock), sblock, 0, c.stream>>>((int64_t)B * nU, P.z.get());
 		if (warm)
-			k_warm_z<<<dim3(ceil_div(nUt, sblock), B), sblock, 0, c.stream>>>(S, n_seg, p0.get(), z.get());
-		ESL_HIP(hipGetLastError());
-		const krylov_result_t kr = krylov_solve(c, S, n_seg, rhs.get(), z.get(), a.opts);
-		resid = kr.resid;
-		iters = kr.iters;
-		cycles = kr.cycles;
-		rep.sets_iterated = kr.sets_iterated;
+			k_warm_z<<<dim3(ceil_div(nUt, sblock), B), sblock, 0, c.stream>>>(S, n_seg, P.p0.get(), P.z.get());
 	}
-	rep.residual = resid;
-	rep.iterations = iters;
-	rep.refinements = cycles;
-	rep.ms_stage[3] = tm_.lap();
+	ESL_HIP(hipGetLastError());
+}
 
-	// ---- absorption masses y[b][j] and the final vector
-	dbuf<double> y, segsum;
-	y.ensure((size_t)B * std::max(Tr, 1));
-	segsum.ensure((size_t)B * std::max(n_seg, 1));
+void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final)
+{
+	graph_t& g = *P.g;
+	const int64_t n = g.n, n_t = g.n_t, T = g.T;
+	const int64_t nU = P.nU;
+	const int n_seg = P.n_seg, nseg1 = std::max(n_seg, 1), Tr = P.Tr;
+	sys_t S = P.sys(B, d_rates);
 	if (Tr)
 	{
-		if (nA)
+		if (P.nA)
 		{
-			const int nbA = ceil_div(nA, sblock);
-			dbuf<double> fpart;
-			fpart.ensure((size_t)B * nbA * Tr);
-			k_exit_flow<<<dim3(nbA, B), sblock, Tr * sizeof(double), c.stream>>>(S, g.rank.get(), T, d_jmap.get(),
-																				 tmask.get(), z.get(), Tr,
-																				 fpart.get());
-			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(fpart.get(), nbA, Tr, y.get(), Tr, 0);
-			ESL_HIP(hipStreamSynchronize(c.stream));
+			const int nbA = ceil_div(P.nA, sblock);
+			k_exit_flow<<<dim3(nbA, B), sblock, Tr * sizeof(double), c.stream>>>(S, g.rank.get(), T, P.d_jmap.get(),
+																				 P.tmask.get(), P.z.get(), Tr,
+																				 P.fpart.get());
+			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(P.fpart.get(), nbA, Tr, P.y.get(), Tr, 0);
 		}
 		else
-			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, y.get());
-		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, y0.get(), y.get());
+			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, P.y.get());
+		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, P.y0.get(), P.y.get());
 	}
 	if (n_seg)
-		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, d_seg_off.get(), z.get(), invd.get(), n_seg,
-														  segsum.get());
-	k_zero<double><<<ceil_div((int64_t)B * n, sblock), sblock, 0, c.stream>>>((int64_t)B * n, a.d_final);
+		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, P.d_seg_off.get(), P.z.get(), P.invd.get(), n_seg,
+														  P.segsum.get());
+	// algorithmic bytes: the dense final vectors written once
+	timed_launch(c, "k_zero<final>", 8.0 * B * n, [&] {
+		k_zero<double><<<ceil_div((int64_t)B * n, sblock), sblock, 0, c.stream>>>((int64_t)B * n, d_final);
+	});
 	if (n_t && Tr)
 		k_final<<<dim3(ceil_div(n_t, sblock), B), sblock, 0, c.stream>>>(
-			n, n_t, g.reordered.get(), g.rank.get(), d_jmap.get(), d_segmap.get(), pos.get(), z.get(), invd.get(),
-			segsum.get(), std::max(n_seg, 1), nU, y.get(), Tr, a.d_final);
+			n, n_t, g.reordered.get(), g.rank.get(), P.d_jmap.get(), P.d_segmap.get(), P.pos.get(), P.z.get(),
+			P.invd.get(), P.segsum.get(), nseg1, nU, P.y.get(), Tr, d_final);
 	ESL_HIP(hipGetLastError());
-	rep.ms_stage[4] = tm_.lap();
-	ESL_HIP(hipStreamSynchronize(c.stream));
-	double tot = 0;
-	for (int i = 0; i < 5; ++i)
-		tot += rep.ms_stage[i];
-	rep.ms_total = tot;
+}
+
+void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o)
+{
+	plan_enqueue_setup(c, P, B, d_rates, o);
+	if (P.nU)
+		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr);
+	plan_enqueue_post(c, P, B, d_rates, d_final);
+}
+
+void plan_reserve(plan_t& P, int B, int m, int precision)
+{
+	const int64_t nU = std::max<int64_t>(P.nU, 1), nUt = std::max<int64_t>(P.nUt, 1);
+	P.p0.ensure((size_t)B * nUt);
+	P.p1.ensure((size_t)B * nUt);
+	P.anchor.ensure((size_t)B * std::max(P.n_seg, 1));
+	P.invd.ensure((size_t)B * nU);
+	P.rhs.ensure((size_t)B * nU);
+	P.z.ensure((size_t)B * nU);
+	P.y.ensure((size_t)B * std::max(P.Tr, 1));
+	P.segsum.ensure((size_t)B * std::max(P.n_seg, 1));
+	P.fpart.ensure((size_t)B * std::max(ceil_div(std::max<int64_t>(P.nA, 1), sblock), 1) * std::max(P.Tr, 1));
+	P.scr.reserve(B, nU, m, precision);
+}
+
+void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
+			  esl_solve_report& rep)
+{
+	esl_solve_opts o = o_in;
+	if (o.tol <= 0)
+		o.tol = 1e-13;
+	if (o.max_iters <= 0)
+		o.max_iters = 20000;
+	timer_t tm_(c);
+	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
+	const bool per_set = P.nU == 0 || P.nU <= per_set_max_unknowns(B);
+	rep.terminals_reached = P.Tr;
+	rep.unknowns = P.nU;
+	rep.reachable = -1;
+	krylov_result_t kr;
+	if (per_set)
+	{
+		plan_reserve(P, B, m, o.precision);
+		const bool use_graph = o.graph != 0;
+		if (use_graph)
+		{
+			const plan_t::gkey_t key{ B, d_rates, d_final, o.warm_pi, o.precision, o.preconditioner, m, o.tol,
+									  o.max_iters };
+			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
+			{
+				P.drop_graph();
+				hipGraph_t graph;
+				if (c.prof.on)
+				{
+					// timing events become nodes of the graph; booked after every replay
+					ESL_HIP(hipStreamSynchronize(c.stream));
+					c.prof.flush();
+					c.prof.capture_events = &P.gev;
+					c.prof.capture_pending = &P.gpend;
+				}
+				ESL_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+				try
+				{
+					plan_enqueue(c, P, B, d_rates, d_final, o);
+				}
+				catch (...)
+				{
+					(void)hipStreamEndCapture(c.stream, &graph);
+					c.prof.capture_events = nullptr;
+					c.prof.capture_pending = nullptr;
+					throw;
+				}
+				c.prof.capture_events = nullptr;
+				c.prof.capture_pending = nullptr;
+				ESL_HIP(hipStreamEndCapture(c.stream, &graph));
+				ESL_HIP(hipGraphInstantiate(&P.gexec, graph, nullptr, nullptr, 0));
+				(void)hipGraphDestroy(graph);
+				P.gkey = key;
+				P.gprof = c.prof.on;
+			}
+			ESL_HIP(hipGraphLaunch(P.gexec, c.stream));
+			ESL_HIP(hipStreamSynchronize(c.stream));
+			if (c.prof.on)
+				c.prof.book(P.gpend);
+		}
+		else
+			plan_enqueue(c, P, B, d_rates, d_final, o);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		kr = P.scr.collect(c, B, o.tol, P.nU > 0);
+	}
+	else
+	{
+		// one large system: device-wide GMRES (host-driven restarts)
+		plan_reserve(P, B, 2, 0);
+		plan_enqueue_setup(c, P, B, d_rates, o);
+		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o);
+		plan_enqueue_post(c, P, B, d_rates, d_final);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+	}
+	rep.residual = kr.resid;
+	rep.iterations = kr.iters;
+	rep.refinements = kr.cycles;
+	rep.sets_iterated = kr.sets_iterated;
+	rep.ms_stage[3] = tm_.lap();
+	rep.ms_total = rep.ms_stage[3];
+}
+
+void solve_run(ctx_t& c, graph_t& g, const solve_args_t& a, esl_solve_report& rep)
+{
+	plan_t P;
+	plan_build(c, g, a.d_x0, P);
+	rep.ms_stage[0] = P.ms[0];
+	rep.ms_stage[1] = P.ms[1];
+	rep.ms_stage[2] = P.ms[2];
+	plan_run(c, P, a.n_sets, a.d_rates, a.d_final, a.opts, rep);
+	rep.ms_total = P.ms[0] + P.ms[1] + P.ms[2] + rep.ms_stage[3];
+}
+
+// ---- opaque plan handles for the C ABI
+void* plan_create(ctx_t& c, graph_t& g, const double* d_x0)
+{
+	auto P = std::make_unique<plan_t>();
+	plan_build(c, g, d_x0, *P);
+	return P.release();
+}
+
+void plan_destroy(void* p) { delete static_cast<plan_t*>(p); }
+
+void plan_solve(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
+				esl_solve_report& rep)
+{
+	plan_run(c, *static_cast<plan_t*>(p), B, d_rates, d_final, o, rep);
+}
+
+void plan_info(const void* p, int64_t info[8])
+{
+	const plan_t& P = *static_cast<const plan_t*>(p);
+	info[0] = P.nU;
+	info[1] = P.nUt;
+	info[2] = P.nA;
+	info[3] = P.Tr;
+	info[4] = P.n_seg;
+	info[5] = P.nbr.nnz;
+	info[6] = P.g ? P.g->n_t : 0;
+	info[7] = 0;
 }
 
 // solution_term and solution_nonterm in the reference layout (one rate set).

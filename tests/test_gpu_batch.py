@@ -1,0 +1,115 @@
+"""Plans, batches, HIP-graph replays and warm starts through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import pipeline as P
+from oracle.bnet import read_bnet_text
+from paraexastolog_amd import model_builder, solve_plan, synth, transition_table, transition_graph, initial_state
+from paraexastolog_amd import solver
+from paraexastolog_amd.surrogate import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def setup_net(ctx, cfg, seed, sets):
+    net, rng = synth.config(cfg, seed)
+    rates = synth.rate_sets(rng, net.N, sets)
+    m = model_builder().parse_model(net.bnet())
+    t = transition_table(ctx, m, rates[0])
+    t.construct_table()
+    g = transition_graph(ctx, t)
+    g.reorganize_vertices()
+    fixed, vals = net.fixed_spec()
+    x0 = initial_state(m.nodes, fixed, vals).state
+    return net, rates, m, t, g, x0
+
+
+@pytest.mark.parametrize("cfg,seed", [("c0", 1), ("c2", 37), ("c2", 0)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_plan_batch_matches_single_solves(ctx, cfg, seed, precision):
+    B = 12
+    net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=precision, tol=1e-12)
+    xb = dout.download()
+    for k in (0, 5, B - 1):
+        s = solver(ctx, t, g, rates[k], x0)
+        s.solve()
+        assert np.abs(xb[k] - s.final_state).sum() <= 1e-10
+        assert abs(xb[k].sum() - 1) < 1e-12
+    assert rep["residual"] <= 1e-12
+
+
+def test_plan_matches_oracle_c0(ctx):
+    B = 4
+    net, rates, m, t, g, x0 = setup_net(ctx, "c0", 3, B)
+    n = 1 << net.N
+    om = read_bnet_text(net.bnet())
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    plan.solve(B, dr.ptr, dout.ptr, precision=0, tol=1e-13)
+    xb = dout.download()
+    ip, rows, _ = P.transition_table(om, rates[0])
+    gr = P.transition_graph(n, ip, rows)
+    for k in range(B):
+        ref = P.solve(n, ip, rows, rates[k], x0, gr, full=True).final_state
+        assert np.abs(xb[k] - ref).sum() <= 1e-12
+
+
+def test_graph_replays_match_eager(ctx):
+    # a captured numeric sequence replayed several times, with eager work and
+    # synchronisations in between; every replay must equal the eager result
+    B = 16
+    net, rates, m, t, g, x0 = setup_net(ctx, "c2", 37, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    d_eager = DeviceArray(ctx, (B, n), np.float64)
+    d_graph = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    plan.solve(B, dr.ptr, d_eager.ptr, precision=1, tol=1e-12)
+    eager = d_eager.download()
+    for rep in range(4):
+        plan.solve(B, dr.ptr, d_graph.ptr, precision=1, tol=1e-12, graph=1)
+        got = d_graph.download()
+        assert np.abs(got - eager).max() <= 1e-13, rep
+        # eager work between replays
+        plan.solve(2, dr.ptr, d_eager.ptr, precision=0, tol=1e-13)
+    # new rates in the same buffer: the replay must pick them up
+    rates2 = rates[::-1].copy()
+    dr.upload(rates2)
+    plan.solve(B, dr.ptr, d_graph.ptr, precision=1, tol=1e-12, graph=1)
+    got = d_graph.download()
+    plan.solve(B, dr.ptr, d_eager.ptr, precision=1, tol=1e-12)
+    assert np.abs(got - d_eager.download()).max() <= 1e-13
+
+
+def test_warm_start_from_exact_and_perturbed(ctx):
+    B = 6
+    net, rates, m, t, g, x0 = setup_net(ctx, "c2", 37, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    cold = plan.solve(B, dr.ptr, dout.ptr, precision=0, tol=1e-13)
+    exact = dout.download()
+    n_t = int(g.sccs_offsets[g.terminals_count])
+    term = g.reordered_vertices[:n_t]
+    warm = np.ascontiguousarray(exact[:, term])
+    dwarm = DeviceArray.from_host(ctx, warm)
+    hot = plan.solve(B, dr.ptr, dout.ptr, precision=0, tol=1e-13, warm=dwarm.ptr)
+    assert np.abs(dout.download() - exact).sum(axis=1).max() <= 1e-12
+    # an exact prediction of the terminal part cannot need more Krylov iterations
+    # than a cold start (the absorption part starts cold in both)
+    assert hot["iterations"] <= cold["iterations"] + 2
+    noisy = warm * np.exp(np.random.default_rng(0).normal(0, 0.3, warm.shape))
+    dwarm.upload(noisy)
+    plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, warm=dwarm.ptr)
+    assert np.abs(dout.download() - exact).sum(axis=1).max() <= 1e-10
