@@ -1246,6 +1246,152 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 	}
 }
 
+
+// ============================================================================
+// device-wide GMRES kernels (nbr_t coupling lists, iterate x unpreconditioned)
+
+__device__ __forceinline__ const double* set_rates(const sys_t& S, int b)
+{
+	return S.rates + (int64_t)b * S.rate_stride;
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg, const X* __restrict__ tin,
+													 X* __restrict__ W)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const int64_t off = (int64_t)b * S.nU;
+	W[off + u] = (X)row_apply<X>(S, G, n_seg, b, u, tin + off, set_rates(S, b));
+}
+
+// r = rhs - M x (FP64) and partial sums of r^2, rhs^2
+__global__ __launch_bounds__(sblock) void k_mk_residual(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
+														const double* __restrict__ x, double* __restrict__ r,
+														double* __restrict__ part)
+{
+	__shared__ double s_red[2][sblock / wave + 1];
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	double rr = 0.0, bb = 0.0;
+	if (u < S.nU)
+	{
+		const int64_t off = (int64_t)b * S.nU;
+		const double bv = rhs[off + u];
+		const double res = bv - row_apply<double>(S, G, n_seg, b, u, x + off, set_rates(S, b));
+		r[off + u] = res;
+		rr = res * res;
+		bb = bv * bv;
+	}
+	rr = block_sum<sblock>(rr, s_red[0]);
+	bb = block_sum<sblock>(bb, s_red[1]);
+	if (threadIdx.x == 0)
+	{
+		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 0] = rr;
+		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 1] = bb;
+	}
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_mk_jacobi(sys_t S, X* __restrict__ v)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nU)
+		return;
+	const int64_t i = (int64_t)b * S.nU + u;
+	v[i] = (X)((double)v[i] * S.invd[i]);
+}
+
+// ILU(0) pivots of one popcount level (see k_gmres_set)
+__global__ __launch_bounds__(sblock) void k_mk_ilu_pivot(sys_t S, nbr_t G, int n_seg, int l, double* __restrict__ ud)
+{
+	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (q >= G.L.off[l + 1])
+		return;
+	const int32_t u = G.lvl[q];
+	const int64_t off = (int64_t)b * S.nU;
+	const double* r = set_rates(S, b);
+	double d = 1.0 / S.invd[off + u];
+	if (!is_anchor(S, n_seg, b, u))
+		for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
+		{
+			const int code = G.nr[k];
+			const int32_t uk = G.nu[k];
+			if ((code & nbr_rev_flag) && !is_anchor(S, n_seg, b, uk))
+				d -= r[code & 63] * r[(code & 63) ^ 1] / ud[off + uk];
+		}
+	ud[off + u] = d;
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_mk_ilu_fwd(sys_t S, nbr_t G, int n_seg, int l,
+													   const double* __restrict__ ud, X* __restrict__ v)
+{
+	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (q >= G.L.off[l + 1])
+		return;
+	const int32_t u = G.lvl[q];
+	if (is_anchor(S, n_seg, b, u))
+		return;
+	const int64_t off = (int64_t)b * S.nU;
+	const double* r = set_rates(S, b);
+	double y = (double)v[off + u];
+	for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
+	{
+		const int32_t uk = G.nu[k];
+		y += r[G.nr[k] & 63] / ud[off + uk] * (double)v[off + uk];
+	}
+	v[off + u] = (X)y;
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_mk_ilu_bwd(sys_t S, nbr_t G, int n_seg, int l,
+													   const double* __restrict__ ud, X* __restrict__ v)
+{
+	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (q >= G.L.off[l + 1])
+		return;
+	const int32_t u = G.lvl[q];
+	if (is_anchor(S, n_seg, b, u))
+		return;
+	const int64_t off = (int64_t)b * S.nU;
+	const double* r = set_rates(S, b);
+	double z = (double)v[off + u];
+	for (int32_t k = G.mid[u]; k < G.ptr[u + 1]; ++k)
+		z += r[G.nr[k] & 63] * (double)v[off + G.nu[k]];
+	v[off + u] = (X)(z / ud[off + u]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_copy(int64_t count, const T* __restrict__ a, T* __restrict__ b)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		b[i] = a[i];
+}
+
+__global__ __launch_bounds__(sblock) void k_axpy1(int64_t count, const double* __restrict__ a, double* __restrict__ x)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		x[i] += a[i];
+}
+
+// z <-> x conversions of the iterate (z = D' x is the convention outside the solvers)
+__global__ __launch_bounds__(sblock) void k_scale_invd(int64_t count, const double* __restrict__ invd,
+													   double* __restrict__ v, int divide)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		v[i] = divide ? v[i] / invd[i] : v[i] * invd[i];
+}
+
 // ============================================================================
 // full mode: terminal kernels of every terminal SCC and the absorption
 // probabilities X[j, c] of every transient state (solution_term /
@@ -1385,143 +1531,6 @@ struct timer_t
 };
 
 
-// GMRES(m) with Jacobi right preconditioning on A_p z = rhs, for all sets at
-// once.  z (FP64) holds the initial guess and receives the solution.  The
-// Krylov basis is kept in T; every restart recomputes the residual in FP64, so
-// with T = float each cycle is one FP32 correction of an FP64 iterate
-// (iterative refinement).
-template <typename T>
-struct gmres_t
-{
-	ctx_t& c;
-	const sys_t& S;
-	int n_seg, m;
-	dbuf<T> V, W;
-	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r;
-	dbuf<int32_t> live;
-	gm_state_t st{};
-	std::vector<double> h_res;
-	int iters = 0, cycles = 0;
-	double final_res = 0.0;
-
-	gmres_t(ctx_t& ctx, const sys_t& sys, int nseg, int restart) : c(ctx), S(sys), n_seg(nseg), m(restart)
-	{
-		const int64_t nU = S.nU, B = S.B;
-		V.ensure((size_t)(m + 1) * nU * B);
-		W.ensure((size_t)nU * B);
-		r.ensure((size_t)nU * B);
-		H.ensure((size_t)B * (m + 1) * m);
-		cs.ensure((size_t)B * m);
-		sn.ensure((size_t)B * m);
-		g.ensure((size_t)B * (m + 1));
-		c1.ensure((size_t)B * (m + 1));
-		c2.ensure((size_t)B * (m + 1));
-		hn.ensure(B);
-		beta.ensure(B);
-		bnorm.ensure(B);
-		res.ensure(B);
-		vscale.ensure(B);
-		live.ensure(B);
-		const int nblk = ceil_div(nU, sblock);
-		part.ensure((size_t)B * nblk * (m + 1) + 16);
-		rr.ensure((size_t)B * 2);
-		st = gm_state_t{ H.get(), cs.get(), sn.get(), g.get(), c1.get(), c2.get(), hn.get(),
-						 beta.get(), bnorm.get(), res.get(), live.get() };
-		h_res.resize(B);
-	}
-
-	double max_res()
-	{
-		read_host(c, h_res, res.get(), (size_t)S.B);
-		double mx = 0.0;
-		for (double v : h_res)
-			mx = std::max(mx, std::isnan(v) ? 1e300 : v);
-		return mx;
-	}
-
-	void solve(const double* rhs, double* z, double tol, int max_iters)
-	{
-		const int64_t nU = S.nU;
-		const int B = S.B;
-		const int nblk = ceil_div(nU, sblock);
-		const dim3 grid(nblk, B);
-		const int64_t vstride = nU * B;
-		const int sb = ceil_div(B, 64);
-		for (;;)
-		{
-			k_residual<<<grid, sblock, 0, c.stream>>>(S, n_seg, rhs, z, r.get(), part.get());
-			k_reduce_parts<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nblk, 2, rr.get(), 2, 0);
-			k_gm_start<<<sb, 64, 0, c.stream>>>(B, m, st, rr.get(), tol, vscale.get());
-			ESL_HIP(hipGetLastError());
-			final_res = max_res();
-			if (final_res <= tol)
-				return;
-			if (iters >= max_iters)
-				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-											   std::to_string(final_res) + " after " + std::to_string(iters) +
-											   " iterations");
-			++cycles;
-			k_cast_scale<T><<<grid, sblock, 0, c.stream>>>(nU, r.get(), vscale.get(), V.get());
-			int k = 0;
-			for (int j = 0; j < m; ++j)
-			{
-				T* Vj = V.get() + (int64_t)j * vstride;
-				// algorithmic bytes: V_j read, W written, 1/D read per unknown and set,
-				// plus the unknown's state, class and mask
-				timed_launch(c, "k_apply", (double)B * nU * (2.0 * sizeof(T) + 8.0) + 12.0 * nU,
-							 [&] { k_apply<T><<<grid, sblock, 0, c.stream>>>(S, n_seg, Vj, W.get()); });
-				// classical Gram-Schmidt, twice
-				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
-				timed_launch(c, "k_dots", dbytes, [&] {
-					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
-				});
-				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
-				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
-					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
-																	 part.get());
-				});
-				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
-				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
-				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
-																 part.get());
-				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
-				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st);
-				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride);
-				ESL_HIP(hipGetLastError());
-				++iters;
-				k = j + 1;
-				const double est = max_res();
-				if (est <= 0.5 * tol || iters >= max_iters)
-					break;
-			}
-			k_gm_solve_y<<<sb, 64, 0, c.stream>>>(B, m, k, st);
-			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, z);
-			ESL_HIP(hipGetLastError());
-		}
-	}
-};
-
-// Systems up to this size per set run as one workgroup per set.  A single
-// large system (few sets) is faster spread over the whole device.
-int64_t per_set_max_unknowns(int B) { return B >= 64 ? 65536 : (B >= 8 ? 16384 : 4096); }
-
-// algorithmic bytes of one per-set GMRES launch: the coupling structure once
-// (row pointers, neighbour index and code), then per set the right-hand side,
-// the iterate (read and written) and 1/D'.
-double gmres_set_bytes(const sys_t& S, int64_t nnz)
-{
-	return (double)S.nU * 12.0 + (double)nnz * 5.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
-}
-
-// Solves A_p z = rhs for every set of S (z holds the initial guess): one
-// workgroup per set when the system is small, device-wide GMRES otherwise.
-struct krylov_result_t
-{
-	double resid = 0.0;
-	int iters = 0, cycles = 0;
-	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
-};
-
 // coupling structure of the per-set solver (see k_nbr_fill)
 struct nbr_store_t
 {
@@ -1572,6 +1581,190 @@ void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
 	st.ready = true;
 }
 
+
+// Device-wide GMRES(m) for systems too large for one workgroup per set: the
+// same method as k_gmres_set (FP64 iterate x, right preconditioning by
+// Jacobi or by the level-scheduled ILU(0), classical Gram-Schmidt twice,
+// FP64 residual at every restart), but every vector operation is a kernel over
+// all unknowns of all sets and the ILU(0) sweeps are one kernel per popcount
+// level.  Restarts and convergence checks go through the host.
+template <typename T>
+struct gmres_t
+{
+	ctx_t& c;
+	const sys_t& S;
+	const nbr_store_t& NB;
+	int n_seg, m;
+	bool ilu;
+	dbuf<T> V, W, t;
+	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r, ud;
+	dbuf<int32_t> live;
+	gm_state_t st{};
+	std::vector<double> h_res;
+	int iters = 0, cycles = 0;
+	double final_res = 0.0;
+
+	gmres_t(ctx_t& ctx, const sys_t& sys, const nbr_store_t& nb, int nseg, int restart, bool use_ilu)
+		: c(ctx), S(sys), NB(nb), n_seg(nseg), m(restart), ilu(use_ilu)
+	{
+		const int64_t nU = S.nU, B = S.B;
+		V.ensure((size_t)(m + 1) * nU * B);
+		W.ensure((size_t)nU * B);
+		t.ensure((size_t)nU * B);
+		r.ensure((size_t)nU * B);
+		ud.ensure((size_t)nU * B);
+		H.ensure((size_t)B * (m + 1) * m);
+		cs.ensure((size_t)B * m);
+		sn.ensure((size_t)B * m);
+		g.ensure((size_t)B * (m + 1));
+		c1.ensure((size_t)B * (m + 1));
+		c2.ensure((size_t)B * (m + 1));
+		hn.ensure(B);
+		beta.ensure(B);
+		bnorm.ensure(B);
+		res.ensure(B);
+		vscale.ensure(B);
+		live.ensure(B);
+		const int nblk = ceil_div(nU, sblock);
+		part.ensure((size_t)B * nblk * (m + 1) + 16);
+		rr.ensure((size_t)B * 2);
+		st = gm_state_t{ H.get(), cs.get(), sn.get(), g.get(), c1.get(), c2.get(), hn.get(),
+						 beta.get(), bnorm.get(), res.get(), live.get() };
+		h_res.resize(B);
+	}
+
+	double max_res()
+	{
+		read_host(c, h_res, res.get(), (size_t)S.B);
+		double mx = 0.0;
+		for (double v : h_res)
+			mx = std::max(mx, std::isnan(v) ? 1e300 : v);
+		return mx;
+	}
+
+	template <typename X>
+	void precondition(X* v) // in place: v <- P^{-1} v
+	{
+		const nbr_t G = NB.view();
+		const int B = S.B;
+		if (!ilu)
+		{
+			k_mk_jacobi<X><<<dim3(ceil_div(S.nU, sblock), B), sblock, 0, c.stream>>>(S, v);
+			return;
+		}
+		for (int l = 0; l < G.L.n; ++l)
+		{
+			const int cnt = G.L.off[l + 1] - G.L.off[l];
+			if (cnt)
+				k_mk_ilu_fwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v);
+		}
+		for (int l = G.L.n - 1; l >= 0; --l)
+		{
+			const int cnt = G.L.off[l + 1] - G.L.off[l];
+			if (cnt)
+				k_mk_ilu_bwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v);
+		}
+	}
+
+	// x: FP64 iterate (in/out)
+	void solve(const double* rhs, double* x, double tol, int max_iters)
+	{
+		const int64_t nU = S.nU;
+		const int B = S.B;
+		const int nblk = ceil_div(nU, sblock);
+		const dim3 grid(nblk, B);
+		const int64_t vstride = nU * B;
+		const int sb = ceil_div(B, 64);
+		const nbr_t G = NB.view();
+		if (ilu)
+			for (int l = 0; l < G.L.n; ++l)
+			{
+				const int cnt = G.L.off[l + 1] - G.L.off[l];
+				if (cnt)
+					k_mk_ilu_pivot<<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get());
+			}
+		for (;;)
+		{
+			k_mk_residual<<<grid, sblock, 0, c.stream>>>(S, G, n_seg, rhs, x, r.get(), part.get());
+			k_reduce_parts<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nblk, 2, rr.get(), 2, 0);
+			k_gm_start<<<sb, 64, 0, c.stream>>>(B, m, st, rr.get(), tol, vscale.get());
+			ESL_HIP(hipGetLastError());
+			final_res = max_res();
+			if (final_res <= tol)
+				return;
+			if (iters >= max_iters)
+				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+											   std::to_string(final_res) + " after " + std::to_string(iters) +
+											   " iterations");
+			++cycles;
+			k_cast_scale<T><<<grid, sblock, 0, c.stream>>>(nU, r.get(), vscale.get(), V.get());
+			int k = 0;
+			for (int j = 0; j < m; ++j)
+			{
+				T* Vj = V.get() + (int64_t)j * vstride;
+				k_copy<T><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, Vj, t.get());
+				precondition<T>(t.get());
+				// algorithmic bytes: t read, W written, coupling structure and 1/D' read
+				timed_launch(c, "k_mk_apply", (double)B * nU * (2.0 * sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz,
+							 [&] { k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get()); });
+				// classical Gram-Schmidt, twice
+				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
+				timed_launch(c, "k_dots", dbytes, [&] {
+					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				});
+				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
+				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
+					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
+																	 part.get());
+				});
+				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
+				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
+																 part.get());
+				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
+				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st);
+				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride);
+				ESL_HIP(hipGetLastError());
+				++iters;
+				k = j + 1;
+				const double est = max_res();
+				if (est <= 0.5 * tol || iters >= max_iters)
+					break;
+			}
+			// x += P^{-1} (sum_i y_i V_i) in FP64
+			k_gm_solve_y<<<sb, 64, 0, c.stream>>>(B, m, k, st);
+			k_zero<double><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get());
+			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, r.get());
+			precondition<double>(r.get());
+			k_axpy1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x);
+			ESL_HIP(hipGetLastError());
+		}
+	}
+};
+
+// Systems up to this size per set run as one workgroup per set.  A single
+// large system (few sets) is faster spread over the whole device.
+int64_t per_set_max_unknowns(int B)
+{
+	return B >= 64 ? (int64_t(1) << 20) : (B >= 16 ? 262144 : (B >= 4 ? 16384 : 4096));
+}
+
+// algorithmic bytes of one per-set GMRES launch: the coupling structure once
+// (row pointers, neighbour index and code), then per set the right-hand side,
+// the iterate (read and written) and 1/D'.
+double gmres_set_bytes(const sys_t& S, int64_t nnz)
+{
+	return (double)S.nU * 12.0 + (double)nnz * 5.0 + (double)S.B * S.nU * (8.0 + 16.0 + 8.0);
+}
+
+// Solves A_p z = rhs for every set of S (z holds the initial guess): one
+// workgroup per set when the system is small, device-wide GMRES otherwise.
+struct krylov_result_t
+{
+	double resid = 0.0;
+	int iters = 0, cycles = 0;
+	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
+};
 
 // scratch of the per-set GMRES kernel, grown on demand and reused
 struct per_set_scratch_t
@@ -1722,7 +1915,7 @@ struct plan_t
 
 
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
-							 const esl_solve_opts& opts)
+							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr)
 {
 	krylov_result_t kr;
 	const int64_t nU = S.nU;
@@ -1733,8 +1926,10 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	if (nU <= per_set_max_unknowns(B))
 	{
 		const int m = std::min(restart, pm_max);
-		nbr_store_t nb;
-		build_nbr(c, S, nb);
+		nbr_store_t local;
+		if (!prebuilt)
+			build_nbr(c, S, local);
+		const nbr_store_t& nb = prebuilt ? *prebuilt : local;
 		per_set_scratch_t sc;
 		sc.reserve(B, nU, m, opts.precision);
 		esl_solve_opts o = opts;
@@ -1745,23 +1940,35 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		ESL_HIP(hipStreamSynchronize(c.stream));
 		kr = sc.collect(c, B, tol, true);
 	}
-	else if (opts.precision == 1)
-	{
-		gmres_t<float> gm(c, S, std::max(n_seg, 1), restart);
-		gm.solve(rhs_p, z_p, tol, max_iters);
-		kr.resid = gm.final_res;
-		kr.iters = gm.iters;
-		kr.cycles = gm.cycles;
-		kr.sets_iterated = gm.cycles > 0 ? B : 0;
-	}
 	else
 	{
-		gmres_t<double> gm(c, S, std::max(n_seg, 1), restart);
-		gm.solve(rhs_p, z_p, tol, max_iters);
-		kr.resid = gm.final_res;
-		kr.iters = gm.iters;
-		kr.cycles = gm.cycles;
-		kr.sets_iterated = gm.cycles > 0 ? B : 0;
+		// device-wide solver: the iterate is x = invd * z inside
+		nbr_store_t local;
+		if (!prebuilt)
+			build_nbr(c, S, local);
+		const nbr_store_t& nb = prebuilt ? *prebuilt : local;
+		const bool ilu = opts.preconditioner != 1;
+		const int64_t cnt = (int64_t)B * nU;
+		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 0);
+		if (opts.precision == 1)
+		{
+			gmres_t<float> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
+			gm.solve(rhs_p, z_p, tol, max_iters);
+			kr.resid = gm.final_res;
+			kr.iters = gm.iters;
+			kr.cycles = gm.cycles;
+		}
+		else
+		{
+			gmres_t<double> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
+			gm.solve(rhs_p, z_p, tol, max_iters);
+			kr.resid = gm.final_res;
+			kr.iters = gm.iters;
+			kr.cycles = gm.cycles;
+		}
+		kr.sets_iterated = kr.cycles > 0 ? B : 0;
+		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 1);
+		ESL_HIP(hipGetLastError());
 	}
 	return kr;
 }
@@ -2112,7 +2319,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		// one large system: device-wide GMRES (host-driven restarts)
 		plan_reserve(P, B, 2, 0);
 		plan_enqueue_setup(c, P, B, d_rates, o);
-		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o);
+		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o, &P.nbr);
 		plan_enqueue_post(c, P, B, d_rates, d_final);
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	}

@@ -113,3 +113,23 @@ def test_warm_start_from_exact_and_perturbed(ctx):
     dwarm.upload(noisy)
     plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, warm=dwarm.ptr)
     assert np.abs(dout.download() - exact).sum(axis=1).max() <= 1e-10
+
+
+@pytest.mark.parametrize("pre", [1, 2])
+def test_device_wide_solver_matches_per_set(ctx, pre):
+    # one rate set with 33k unknowns runs device-wide (kernels per ILU level);
+    # 16 sets run one workgroup per set: same answers
+    B = 16
+    net, rates, m, t, g, x0 = setup_net(ctx, "c2", 37, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    assert plan.info()["unknowns"] > 16384
+    d16 = DeviceArray(ctx, (B, n), np.float64)
+    plan.solve(B, dr.ptr, d16.ptr, precision=0, tol=1e-13, preconditioner=pre)
+    d1 = DeviceArray(ctx, (1, n), np.float64)
+    rep = plan.solve(1, dr.offset(2 * net.N * 3), d1.ptr, precision=0, tol=1e-13, preconditioner=pre)
+    assert np.abs(d1.download()[0] - d16.download()[3]).sum() <= 1e-11
+    if pre == 2:
+        assert rep["iterations"] < 500
