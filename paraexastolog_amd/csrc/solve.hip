@@ -941,8 +941,8 @@ struct pgm_scratch_t
 {
 	double* r;	   // [B][nU] residual / FP64 correction
 	double* udiag; // [B][nU] ILU(0) pivots
-	void* t;	   // [B][nU] preconditioned basis vector (basis precision)
-	void* V;	   // [B][m+1][nU]
+	double* t;	   // [B][nU] preconditioned basis vector (FP64: the preconditioner is exact to FP64)
+	void* V;	   // [B][m+1][nU] Krylov basis (FP64 or FP32)
 };
 
 // sum over the workgroup of one value per thread, result broadcast
@@ -1044,7 +1044,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 	double* __restrict__ rr_ = sc.r + off;
 	double* __restrict__ ud = sc.udiag + off;
 	const double* __restrict__ ib = S.invd + off;
-	T* __restrict__ t = (T*)sc.t + off;
+	double* __restrict__ t = sc.t + off; // preconditioned vector, always FP64
 	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
 	for (int i = threadIdx.x; i < 2 * S.N; i += pblock)
 		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
@@ -1120,20 +1120,20 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 			if (ILU)
 			{
 				for (int64_t u = threadIdx.x; u < nU; u += pblock)
-					t[u] = Vj[u];
+					t[u] = (double)Vj[u];
 				__threadfence_block();
 				__syncthreads();
-				ilu_apply<T>(S, G, n_seg, b, ud, s_rates, t);
+				ilu_apply<double>(S, G, n_seg, b, ud, s_rates, t);
 			}
 			else
 			{
 				for (int64_t u = threadIdx.x; u < nU; u += pblock)
-					t[u] = (T)((double)Vj[u] * ib[u]);
+					t[u] = (double)Vj[u] * ib[u];
 				__threadfence_block();
 				__syncthreads();
 			}
 			for (int64_t u = threadIdx.x; u < nU; u += pblock)
-				W[u] = (T)row_apply<T>(S, G, n_seg, b, u, t, s_rates);
+				W[u] = (T)row_apply<double>(S, G, n_seg, b, u, t, s_rates);
 			__threadfence_block();
 			__syncthreads();
 			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
@@ -1256,7 +1256,7 @@ __device__ __forceinline__ const double* set_rates(const sys_t& S, int b)
 }
 
 template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg, const X* __restrict__ tin,
+__global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg, const double* __restrict__ tin,
 													 X* __restrict__ W)
 {
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg
 	if (u >= S.nU)
 		return;
 	const int64_t off = (int64_t)b * S.nU;
-	W[off + u] = (X)row_apply<X>(S, G, n_seg, b, u, tin + off, set_rates(S, b));
+	W[off + u] = (X)row_apply<double>(S, G, n_seg, b, u, tin + off, set_rates(S, b));
 }
 
 // r = rhs - M x (FP64) and partial sums of r^2, rhs^2
@@ -1369,11 +1369,11 @@ __global__ __launch_bounds__(sblock) void k_mk_ilu_bwd(sys_t S, nbr_t G, int n_s
 }
 
 template <typename T>
-__global__ __launch_bounds__(sblock) void k_copy(int64_t count, const T* __restrict__ a, T* __restrict__ b)
+__global__ __launch_bounds__(sblock) void k_copy(int64_t count, const T* __restrict__ a, double* __restrict__ b)
 {
 	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	if (i < count)
-		b[i] = a[i];
+		b[i] = (double)a[i];
 }
 
 __global__ __launch_bounds__(sblock) void k_axpy1(int64_t count, const double* __restrict__ a, double* __restrict__ x)
@@ -1596,7 +1596,8 @@ struct gmres_t
 	const nbr_store_t& NB;
 	int n_seg, m;
 	bool ilu;
-	dbuf<T> V, W, t;
+	dbuf<T> V, W;
+	dbuf<double> t; // preconditioned vector, FP64
 	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r, ud;
 	dbuf<int32_t> live;
 	gm_state_t st{};
@@ -1703,9 +1704,9 @@ struct gmres_t
 			{
 				T* Vj = V.get() + (int64_t)j * vstride;
 				k_copy<T><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, Vj, t.get());
-				precondition<T>(t.get());
+				precondition<double>(t.get());
 				// algorithmic bytes: t read, W written, coupling structure and 1/D' read
-				timed_launch(c, "k_mk_apply", (double)B * nU * (2.0 * sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz,
+				timed_launch(c, "k_mk_apply", (double)B * nU * (8.0 + sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz,
 							 [&] { k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get()); });
 				// classical Gram-Schmidt, twice
 				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
@@ -1769,10 +1770,10 @@ struct krylov_result_t
 // scratch of the per-set GMRES kernel, grown on demand and reused
 struct per_set_scratch_t
 {
-	dbuf<double> r, udiag, res;
+	dbuf<double> r, udiag, res, t;
 	dbuf<int32_t> its, cyc;
-	dbuf<float> Vf, tf;
-	dbuf<double> Vd, td;
+	dbuf<float> Vf;
+	dbuf<double> Vd;
 	void reserve(int B, int64_t nU, int m, int precision)
 	{
 		nU = std::max<int64_t>(nU, 1);
@@ -1781,16 +1782,11 @@ struct per_set_scratch_t
 		res.ensure(B);
 		its.ensure(B);
 		cyc.ensure(B);
+		t.ensure((size_t)B * nU);
 		if (precision == 1)
-		{
 			Vf.ensure((size_t)B * (m + 1) * nU);
-			tf.ensure((size_t)B * nU);
-		}
 		else
-		{
 			Vd.ensure((size_t)B * (m + 1) * nU);
-			td.ensure((size_t)B * nU);
-		}
 	}
 	krylov_result_t collect(ctx_t& c, int B, double tol, bool any)
 	{
@@ -1830,7 +1826,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const double by = gmres_set_bytes(S, nb.nnz);
 	if (o.precision == 1)
 	{
-		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.tf.get(), sc.Vf.get() };
+		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
 		timed_launch(c, ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<float, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
@@ -1842,7 +1838,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	}
 	else
 	{
-		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.td.get(), sc.Vd.get() };
+		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vd.get() };
 		timed_launch(c, ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<double, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
