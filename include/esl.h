@@ -151,6 +151,7 @@ typedef struct
 	int64_t unknowns;	 /* unknowns of the linear systems (terminal kernels + ambiguous transient states) */
 	int terminals_reached; /* terminal SCCs reachable from supp(x0) */
 	int sets_iterated;	 /* rate sets whose initial iterate (zero, or the warm start) missed the tolerance */
+	int fp64_fallback;	 /* precision 1: rate sets whose FP32 cycles stalled and that were finished in FP64 */
 	double ms_total;	 /* device time of the call */
 	double ms_stage[8];	 /* reach, terminal masks, setup, terminal solve, transient solve, final, -, - */
 } esl_solve_report;

@@ -44,6 +44,7 @@ class SolveReport(C.Structure):
         ("unknowns", C.c_int64),
         ("terminals_reached", C.c_int),
         ("sets_iterated", C.c_int),
+        ("fp64_fallback", C.c_int),
         ("ms_total", C.c_double),
         ("ms_stage", C.c_double * 8),
     ]
@@ -56,6 +57,7 @@ class SolveReport(C.Structure):
             "unknowns": self.unknowns,
             "terminals_reached": self.terminals_reached,
             "sets_iterated": self.sets_iterated,
+            "fp64_fallback": self.fp64_fallback,
             "ms_total": self.ms_total,
             "ms_stage": list(self.ms_stage)[:5],
         }

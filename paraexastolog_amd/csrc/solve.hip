@@ -438,11 +438,12 @@ __global__ __launch_bounds__(sblock) void k_dots(int64_t nU, int B, int K, const
 	}
 }
 
-// W -= sum_i c[b][i] V_i ; partial sums of |W|^2
+// W -= sum_i c[b][i] V_i (c[b] at c + b * cstride); partial sums of |W|^2
 template <typename T>
 __global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __restrict__ W,
 														const T* __restrict__ V, int64_t vstride,
-														const double* __restrict__ c, double* __restrict__ part)
+														const double* __restrict__ c, int64_t cstride,
+														double* __restrict__ part)
 {
 	__shared__ double s_red[sblock / wave + 1];
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __
 	if (u < nU)
 	{
 		double w = (double)W[(int64_t)b * nU + u];
-		const double* cb = c + (int64_t)b * K;
+		const double* cb = c + (int64_t)b * cstride;
 		for (int i = 0; i < K; ++i)
 			w -= cb[i] * (double)V[i * vstride + (int64_t)b * nU + u];
 		const T wt = (T)w;
@@ -935,6 +936,7 @@ struct pgm_out_t
 	int32_t* iters;	  // [B]
 	int32_t* cycles;  // [B]
 	double* res;	  // [B] final relative FP64 residual
+	int32_t* stalled; // [B] 1 when FP32 cycles stopped reducing the FP64 residual
 };
 
 struct pgm_scratch_t
@@ -1079,8 +1081,8 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 		}
 	}
 
-	int iters = 0, cycles = 0;
-	double rel = 0.0;
+	int iters = 0, cycles = 0, stalled = 0;
+	double rel = 0.0, rel_prev = 0.0;
 	for (;;)
 	{
 		double rr = 0.0, bb = 0.0;
@@ -1098,6 +1100,22 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 		rel = beta / bn;
 		if (rel <= tol || iters >= max_iters)
 			break;
+		if (sizeof(T) == 4 && cycles > 0 && !(rel < 0.9 * rel_prev))
+		{
+			// an FP32 cycle that does not reduce the FP64 residual: undo it if it
+			// made things worse and leave the set to the FP64 solver
+			if (!(rel <= rel_prev))
+			{
+				for (int64_t u = threadIdx.x; u < nU; u += pblock)
+					x[u] -= t[u];
+				rel = rel_prev;
+			}
+			stalled = 1;
+			__threadfence_block();
+			__syncthreads();
+			break;
+		}
+		rel_prev = rel;
 		++cycles;
 		const double inv_beta = 1.0 / beta;
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
@@ -1220,19 +1238,20 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 				s_c[i] = d != 0.0 ? tt / d : 0.0;
 			}
 		__syncthreads();
+		// the correction stays in t until the next residual has been checked
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
 		{
 			double acc = 0.0;
 			for (int i = 0; i < k; ++i)
 				acc += s_c[i] * (double)V[(int64_t)i * nU + u];
-			rr_[u] = ILU ? acc : acc * ib[u];
+			t[u] = ILU ? acc : acc * ib[u];
 		}
 		__threadfence_block();
 		__syncthreads();
 		if (ILU)
-			ilu_apply<double>(S, G, n_seg, b, ud, s_rates, rr_);
+			ilu_apply<double>(S, G, n_seg, b, ud, s_rates, t);
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
-			x[u] += rr_[u];
+			x[u] += t[u];
 		__threadfence_block();
 		__syncthreads();
 	}
@@ -1243,6 +1262,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 		out.iters[b] = iters;
 		out.cycles[b] = cycles;
 		out.res[b] = rel;
+		out.stalled[b] = stalled;
 	}
 }
 
@@ -1381,6 +1401,25 @@ __global__ __launch_bounds__(sblock) void k_axpy1(int64_t count, const double* _
 	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	if (i < count)
 		x[i] += a[i];
+}
+
+// x <- x - a (undoes a correction); a is kept for that purpose: b <- a first
+__global__ __launch_bounds__(sblock) void k_axpy1_keep(int64_t count, const double* __restrict__ a,
+													   double* __restrict__ x, double* __restrict__ keep)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+	{
+		keep[i] = a[i];
+		x[i] += a[i];
+	}
+}
+
+__global__ __launch_bounds__(sblock) void k_sub1(int64_t count, const double* __restrict__ a, double* __restrict__ x)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count)
+		x[i] -= a[i];
 }
 
 // z <-> x conversions of the iterate (z = D' x is the convention outside the solvers)
@@ -1667,9 +1706,12 @@ struct gmres_t
 		}
 	}
 
-	// x: FP64 iterate (in/out)
-	void solve(const double* rhs, double* x, double tol, int max_iters)
+	// x: FP64 iterate (in/out).  Returns false when an FP32 cycle failed to
+	// reduce the FP64 residual (the cycle is undone if it increased it); the
+	// caller then continues in FP64 from x.
+	bool solve(const double* rhs, double* x, double tol, int max_iters)
 	{
+		double prev = 0.0;
 		const int64_t nU = S.nU;
 		const int B = S.B;
 		const int nblk = ceil_div(nU, sblock);
@@ -1692,7 +1734,18 @@ struct gmres_t
 			ESL_HIP(hipGetLastError());
 			final_res = max_res();
 			if (final_res <= tol)
-				return;
+				return true;
+			if (sizeof(T) == 4 && cycles > 0 && !(final_res < 0.9 * prev))
+			{
+				if (!(final_res <= prev))
+				{
+					k_sub1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, t.get(), x);
+					ESL_HIP(hipGetLastError());
+					final_res = prev;
+				}
+				return false;
+			}
+			prev = final_res;
 			if (iters >= max_iters)
 				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
 											   std::to_string(final_res) + " after " + std::to_string(iters) +
@@ -1716,12 +1769,12 @@ struct gmres_t
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
 				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
 					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
-																	 part.get());
+																	 m + 1, part.get());
 				});
 				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
 				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
-																 part.get());
+																 m + 1, part.get());
 				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
 				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st);
 				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride);
@@ -1737,7 +1790,10 @@ struct gmres_t
 			k_zero<double><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get());
 			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, r.get());
 			precondition<double>(r.get());
-			k_axpy1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x);
+			if (sizeof(T) == 4) // keep the correction in t until the next residual has been checked
+				k_axpy1_keep<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x, t.get());
+			else
+				k_axpy1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x);
 			ESL_HIP(hipGetLastError());
 		}
 	}
@@ -1765,13 +1821,15 @@ struct krylov_result_t
 	double resid = 0.0;
 	int iters = 0, cycles = 0;
 	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
+	int fp64_fallback = 0; // sets finished in FP64 after their FP32 cycles stalled
+	int missed = 0;		   // sets above the tolerance
 };
 
 // scratch of the per-set GMRES kernel, grown on demand and reused
 struct per_set_scratch_t
 {
 	dbuf<double> r, udiag, res, t;
-	dbuf<int32_t> its, cyc;
+	dbuf<int32_t> its, cyc, stl;
 	dbuf<float> Vf;
 	dbuf<double> Vd;
 	void reserve(int B, int64_t nU, int m, int precision)
@@ -1782,13 +1840,14 @@ struct per_set_scratch_t
 		res.ensure(B);
 		its.ensure(B);
 		cyc.ensure(B);
+		stl.ensure(B);
 		t.ensure((size_t)B * nU);
 		if (precision == 1)
 			Vf.ensure((size_t)B * (m + 1) * nU);
 		else
 			Vd.ensure((size_t)B * (m + 1) * nU);
 	}
-	krylov_result_t collect(ctx_t& c, int B, double tol, bool any)
+	krylov_result_t collect(ctx_t& c, int B, double tol, bool any, bool check = true)
 	{
 		krylov_result_t kr;
 		if (!any)
@@ -1804,8 +1863,9 @@ struct per_set_scratch_t
 			kr.iters = std::max(kr.iters, h_its[b]);
 			kr.cycles = std::max(kr.cycles, h_cyc[b]);
 			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
+			kr.missed += (h_res[b] <= tol) ? 0 : 1;
 		}
-		if (!(kr.resid <= tol))
+		if (check && !(kr.resid <= tol))
 			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
 										   std::to_string(kr.resid) + " after " + std::to_string(kr.iters) +
 										   " iterations");
@@ -1822,7 +1882,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const int nseg1 = std::max(n_seg, 1);
 	const bool ilu = o.preconditioner != 1; // ILU(0) unless Jacobi is asked for
 	const nbr_t G = nb.view();
-	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get() };
+	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get(), sc.stl.get() };
 	const double by = gmres_set_bytes(S, nb.nnz);
 	if (o.precision == 1)
 	{
@@ -1849,6 +1909,31 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 		});
 	}
 	ESL_HIP(hipGetLastError());
+}
+
+// Reads the per-set results of a launch_per_set.  In mixed precision, sets that
+// missed the tolerance (their FP32 cycles stalled) are finished by the FP64
+// kernel from their current iterate; sets that already converged leave it at
+// their first residual check.
+krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs,
+							   double* z, esl_solve_opts o, per_set_scratch_t& sc)
+{
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	const int B = S.B;
+	krylov_result_t kr = sc.collect(c, B, o.tol, true, o.precision != 1);
+	if (o.precision != 1 || kr.missed == 0)
+		return kr;
+	o.precision = 0;
+	sc.reserve(B, S.nU, o.restart, 0);
+	launch_per_set(c, S, nb, n_seg, rhs, z, o, sc);
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	const krylov_result_t k2 = sc.collect(c, B, o.tol, true, true);
+	kr.fp64_fallback = kr.missed;
+	kr.missed = 0;
+	kr.resid = k2.resid;
+	kr.iters += k2.iters;
+	kr.cycles += k2.cycles;
+	return kr;
 }
 
 // everything a solve needs that does not depend on the rates
@@ -1933,8 +2018,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		o.max_iters = max_iters;
 		o.restart = m;
 		launch_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc);
-		ESL_HIP(hipStreamSynchronize(c.stream));
-		kr = sc.collect(c, B, tol, true);
+		kr = finish_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc);
 	}
 	else
 	{
@@ -1946,21 +2030,25 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		const bool ilu = opts.preconditioner != 1;
 		const int64_t cnt = (int64_t)B * nU;
 		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 0);
+		bool done = false;
 		if (opts.precision == 1)
 		{
 			gmres_t<float> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			gm.solve(rhs_p, z_p, tol, max_iters);
+			done = gm.solve(rhs_p, z_p, tol, max_iters);
 			kr.resid = gm.final_res;
 			kr.iters = gm.iters;
 			kr.cycles = gm.cycles;
+			if (!done)
+				kr.fp64_fallback = B;
 		}
-		else
+		if (!done)
 		{
+			// FP64 from the current iterate (also what an FP32 solve that stalled falls back to)
 			gmres_t<double> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			gm.solve(rhs_p, z_p, tol, max_iters);
+			gm.solve(rhs_p, z_p, tol, std::max(max_iters - kr.iters, 1));
 			kr.resid = gm.final_res;
-			kr.iters = gm.iters;
-			kr.cycles = gm.cycles;
+			kr.iters += gm.iters;
+			kr.cycles += gm.cycles;
 		}
 		kr.sets_iterated = kr.cycles > 0 ? B : 0;
 		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 1);
@@ -2308,7 +2396,15 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		else
 			plan_enqueue(c, P, B, d_rates, d_final, o);
 		ESL_HIP(hipStreamSynchronize(c.stream));
-		kr = P.scr.collect(c, B, o.tol, P.nU > 0);
+		if (P.nU > 0)
+		{
+			esl_solve_opts of = o;
+			of.restart = m;
+			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), of, P.scr);
+			if (kr.fp64_fallback) // the final vectors are rebuilt from the FP64 iterates
+				plan_enqueue_post(c, P, B, d_rates, d_final);
+			ESL_HIP(hipStreamSynchronize(c.stream));
+		}
 	}
 	else
 	{
@@ -2323,6 +2419,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	rep.iterations = kr.iters;
 	rep.refinements = kr.cycles;
 	rep.sets_iterated = kr.sets_iterated;
+	rep.fp64_fallback = kr.fp64_fallback;
 	rep.ms_stage[3] = tm_.lap();
 	rep.ms_total = rep.ms_stage[3];
 }
