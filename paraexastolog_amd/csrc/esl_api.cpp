@@ -116,6 +116,10 @@ esl_status esl_ctx_create(int device, esl_ctx** out)
 		ctx->c.device = device;
 		ESL_HIP(hipSetDevice(device));
 		ESL_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+		int lds = 0;
+		ESL_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+		if (lds > 0)
+			ctx->c.lds_per_block = lds;
 		for (auto& e : ctx->c.ev)
 			ESL_HIP(hipEventCreate(&e));
 		ctx->c.flags.ensure(64);

@@ -136,6 +136,8 @@ struct prof_t
 				k.ms += ms;
 				k.bytes += p.bytes;
 			}
+			else
+				(void)hipGetLastError(); // an untimed pair is skipped, not left as the next call's error
 		}
 	}
 	void flush()
@@ -154,6 +156,7 @@ struct prof_t
 struct ctx_t
 {
 	int device = 0;
+	int lds_per_block = 65536; // dynamic + static LDS a workgroup may use on this device
 	hipStream_t stream = nullptr;
 	std::string last_error;
 	hipEvent_t ev[16] = {};
@@ -163,6 +166,23 @@ struct ctx_t
 	int32_t* h_flags = nullptr; // pinned mirror
 	prof_t prof;
 };
+
+// Appends an event-record node to the graph being captured on `s`, after the
+// nodes captured so far, and makes it the dependency of what follows.
+inline void record_event_node(hipStream_t s, hipEvent_t e)
+{
+	hipStreamCaptureStatus st;
+	unsigned long long id = 0;
+	hipGraph_t g = nullptr;
+	const hipGraphNode_t* deps = nullptr;
+	size_t nd = 0;
+	ESL_HIP(hipStreamGetCaptureInfo_v2(s, &st, &id, &g, &deps, &nd));
+	if (st != hipStreamCaptureStatusActive)
+		throw error(ESL_E_STATE, "stream is not capturing");
+	hipGraphNode_t node;
+	ESL_HIP(hipGraphAddEventRecordNode(&node, g, deps, nd, e));
+	ESL_HIP(hipStreamUpdateCaptureDependencies(s, &node, 1, hipStreamSetCaptureDependencies));
+}
 
 // Launch `launch` (a lambda issuing one kernel on c.stream); with profiling on,
 // bracket it with events and book `bytes` algorithmic bytes under `name`.
@@ -175,14 +195,19 @@ inline void timed_launch(ctx_t& c, const char* name, double bytes, F&& launch)
 		return;
 	}
 	hipEvent_t a = c.prof.next_event(), b = c.prof.next_event();
-	ESL_HIP(hipEventRecord(a, c.stream));
-	launch();
-	ESL_HIP(hipEventRecord(b, c.stream));
 	if (c.prof.capture_pending)
 	{
+		// an hipEventRecord captured from the stream cannot be timed on replay;
+		// explicit event-record nodes can
+		record_event_node(c.stream, a);
+		launch();
+		record_event_node(c.stream, b);
 		c.prof.capture_pending->push_back({ name, a, b, bytes });
 		return;
 	}
+	ESL_HIP(hipEventRecord(a, c.stream));
+	launch();
+	ESL_HIP(hipEventRecord(b, c.stream));
 	c.prof.pending.push_back({ name, a, b, bytes });
 	if (c.prof.pending.size() > 4096)
 	{

@@ -1266,6 +1266,457 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 	}
 }
 
+// ============================================================================
+// LDS-resident variant of k_gmres_set for systems of up to 2048 unknowns per
+// set (the common case: one terminal SCC of ~1000 states, many rate sets).
+// The ILU(0) sweeps are chains of dependent reads, one chain per popcount
+// level; from global memory every level pays several L2 round trips.  Here the
+// coupling structure (shared by all sets, 16-bit indices), the inverse pivots
+// and the vector being preconditioned live in LDS, so a level costs LDS
+// latency plus one workgroup barrier.  Each thread owns the unknowns
+// u = threadIdx.x + i * pblock (i < UPT): the Arnoldi vector w, the basis
+// entries and D' of its own rows stay in registers, and the basis in global
+// memory is only ever touched by the thread that owns the row, so it needs no
+// barrier.  Same method and results as k_gmres_set (ILU(0) only).
+
+struct lds_plan_t
+{
+	int off_H, off_cs, off_sn, off_g, off_c, off_part, off_t, off_ud, off_lvl, off_ptr, off_nlow, off_nu, off_nc,
+		off_anc, bytes;
+};
+
+__host__ __device__ inline int lds_take(int& o, int64_t bytes)
+{
+	const int at = o;
+	o += (int)((bytes + 15) / 16 * 16);
+	return at;
+}
+
+// dynamic LDS carve-up for nU unknowns, nnz couplings, restart m, vectors of psize bytes
+inline lds_plan_t lds_plan(int64_t nU, int64_t nnz, int m, int psize)
+{
+	lds_plan_t p{};
+	int o = 0;
+	p.off_H = lds_take(o, 8 * (int64_t)(m + 1) * m);
+	p.off_cs = lds_take(o, 8 * (int64_t)m);
+	p.off_sn = lds_take(o, 8 * (int64_t)m);
+	p.off_g = lds_take(o, 8 * (int64_t)(m + 1));
+	p.off_c = lds_take(o, 8 * (int64_t)(m + 1));
+	p.off_part = lds_take(o, 8 * (int64_t)(m + 1) * pwaves);
+	p.off_t = lds_take(o, psize * nU);
+	p.off_ud = lds_take(o, psize * nU);
+	p.off_lvl = lds_take(o, 2 * nU);
+	p.off_ptr = lds_take(o, 2 * (nU + 1));
+	p.off_nlow = lds_take(o, nU);
+	p.off_nu = lds_take(o, 2 * nnz);
+	p.off_nc = lds_take(o, nnz);
+	p.off_anc = lds_take(o, 4 * ((nU + 31) / 32));
+	p.bytes = o;
+	return p;
+}
+
+constexpr int lds_max_unknowns = 2048;
+constexpr int lds_static_bytes = 1024; // upper bound of the kernel's static LDS
+
+struct lds_view_t
+{
+	const uint16_t* lvl;
+	const uint16_t* ptr;
+	const uint8_t* nlow;
+	const uint16_t* nu;
+	const uint8_t* nc;
+	const uint32_t* anc;
+	const double* rates;
+	__device__ bool anchor(int u) const { return (anc[u >> 5] >> (u & 31)) & 1u; }
+};
+
+// in place on LDS: t <- P^{-1} t (ILU(0), inverse pivots iu); ends with a barrier
+template <typename P>
+__device__ void ilu_lds(const levels_t& L, const lds_view_t& v, const P* __restrict__ iu, P* __restrict__ t)
+{
+	for (int l = 0; l < L.n; ++l)
+	{
+		const int a = L.off[l], e = L.off[l + 1];
+		if (a == e)
+			continue;
+		for (int q = a + threadIdx.x; q < e; q += pblock)
+		{
+			const int u = v.lvl[q];
+			if (v.anchor(u))
+				continue;
+			double y = (double)t[u];
+			const int k0 = v.ptr[u], k1 = k0 + v.nlow[u];
+			for (int k = k0; k < k1; ++k)
+			{
+				const int uk = v.nu[k];
+				y += v.rates[v.nc[k] & 63] * (double)iu[uk] * (double)t[uk];
+			}
+			t[u] = (P)y;
+		}
+		__syncthreads();
+	}
+	for (int l = L.n - 1; l >= 0; --l)
+	{
+		const int a = L.off[l], e = L.off[l + 1];
+		if (a == e)
+			continue;
+		for (int q = a + threadIdx.x; q < e; q += pblock)
+		{
+			const int u = v.lvl[q];
+			if (v.anchor(u))
+				continue;
+			double z = (double)t[u];
+			const int k0 = v.ptr[u] + v.nlow[u], k1 = v.ptr[u + 1];
+			for (int k = k0; k < k1; ++k)
+				z += v.rates[v.nc[k] & 63] * (double)t[v.nu[k]];
+			t[u] = (P)(z * (double)iu[u]);
+		}
+		__syncthreads();
+	}
+}
+
+// row u of M applied to a vector y held in LDS or global memory (FP64 sum)
+template <typename Y>
+__device__ __forceinline__ double row_lds(const lds_view_t& v, int u, double d, const Y* __restrict__ y)
+{
+	if (v.anchor(u))
+		return (double)y[u];
+	double acc = d * (double)y[u];
+	const int k1 = v.ptr[u + 1];
+	for (int k = v.ptr[u]; k < k1; ++k)
+		acc -= v.rates[v.nc[k] & 63] * (double)y[v.nu[k]];
+	return acc;
+}
+
+template <typename T, int UPT>
+__global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
+													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
+													  int max_iters, pgm_out_t out, lds_plan_t lp)
+{
+	using P = T; // precision of the LDS vectors (preconditioned vector, inverse pivots)
+	extern __shared__ __align__(16) unsigned char smem[];
+	double* s_H = (double*)(smem + lp.off_H);
+	double* s_cs = (double*)(smem + lp.off_cs);
+	double* s_sn = (double*)(smem + lp.off_sn);
+	double* s_g = (double*)(smem + lp.off_g);
+	double* s_c = (double*)(smem + lp.off_c);
+	double* s_part = (double*)(smem + lp.off_part);
+	P* s_t = (P*)(smem + lp.off_t);
+	P* s_iu = (P*)(smem + lp.off_ud);
+	uint16_t* s_lvl = (uint16_t*)(smem + lp.off_lvl);
+	uint16_t* s_ptr = (uint16_t*)(smem + lp.off_ptr);
+	uint8_t* s_nlow = (uint8_t*)(smem + lp.off_nlow);
+	uint16_t* s_nu = (uint16_t*)(smem + lp.off_nu);
+	uint8_t* s_nc = (uint8_t*)(smem + lp.off_nc);
+	uint32_t* s_anc = (uint32_t*)(smem + lp.off_anc);
+	__shared__ double s_red[pwaves];
+	__shared__ double s_scalar[4];
+	__shared__ double s_rates[64];
+
+	const int b = blockIdx.x;
+	const int tid = threadIdx.x;
+	const int nU = (int)S.nU;
+	const int64_t off = (int64_t)b * nU;
+	const double* __restrict__ rb = rhs + off;
+	double* __restrict__ x = zio + off; // z = D' x on entry and exit, x inside
+	const double* __restrict__ ib = S.invd + off;
+	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
+	const int mh = m + 1; // column stride of s_H
+
+	// ---- coupling structure, rates and anchor flags into LDS
+	const int nnz = G.ptr[nU];
+	for (int q = tid; q < nU; q += pblock)
+	{
+		s_lvl[q] = (uint16_t)G.lvl[q];
+		s_ptr[q] = (uint16_t)G.ptr[q];
+		s_nlow[q] = (uint8_t)(G.mid[q] - G.ptr[q]);
+	}
+	if (tid == 0)
+		s_ptr[nU] = (uint16_t)nnz;
+	for (int k = tid; k < nnz; k += pblock)
+	{
+		s_nu[k] = (uint16_t)G.nu[k];
+		s_nc[k] = G.nr[k];
+	}
+	for (int i = tid; i < 2 * S.N; i += pblock)
+		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
+	for (int w = tid; w < (nU + 31) / 32; w += pblock)
+	{
+		uint32_t bits = 0;
+		for (int i = 0; i < 32 && w * 32 + i < nU; ++i)
+			bits |= is_anchor(S, n_seg, b, w * 32 + i) ? (1u << i) : 0u;
+		s_anc[w] = bits;
+	}
+	double dg[UPT]; // D' of the own rows
+#pragma unroll
+	for (int i = 0; i < UPT; ++i)
+	{
+		const int u = tid + i * pblock;
+		dg[i] = 1.0;
+		if (u < nU)
+		{
+			dg[i] = 1.0 / ib[u];
+			x[u] *= ib[u];
+		}
+	}
+	__threadfence_block();
+	__syncthreads();
+	const lds_view_t lv{ s_lvl, s_ptr, s_nlow, s_nu, s_nc, s_anc, s_rates };
+
+	// ---- inverse ILU(0) pivots, level by level
+	for (int l = 0; l < G.L.n; ++l)
+	{
+		const int a = G.L.off[l], e = G.L.off[l + 1];
+		if (a == e)
+			continue;
+		for (int q = a + tid; q < e; q += pblock)
+		{
+			const int u = s_lvl[q];
+			double d = 1.0 / ib[u];
+			if (!lv.anchor(u))
+			{
+				const int k0 = s_ptr[u], k1 = k0 + s_nlow[u];
+				for (int k = k0; k < k1; ++k)
+				{
+					const int code = s_nc[k];
+					const int uk = s_nu[k];
+					if ((code & nbr_rev_flag) && !lv.anchor(uk))
+						d -= s_rates[code & 63] * s_rates[(code & 63) ^ 1] * (double)s_iu[uk];
+				}
+			}
+			s_iu[u] = (P)(1.0 / d);
+		}
+		__syncthreads();
+	}
+
+	int iters = 0, cycles = 0, stalled = 0;
+	double rel = 0.0, rel_prev = 0.0;
+	double corr[UPT]; // last correction of the own rows (undone when an FP32 cycle made things worse)
+#pragma unroll
+	for (int i = 0; i < UPT; ++i)
+		corr[i] = 0.0;
+	for (;;)
+	{
+		// FP64 residual of the own rows
+		double ro[UPT];
+		double rr = 0.0, bb = 0.0;
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int u = tid + i * pblock;
+			ro[i] = 0.0;
+			if (u < nU)
+			{
+				ro[i] = rb[u] - row_lds<double>(lv, u, dg[i], x);
+				rr += ro[i] * ro[i];
+				bb += rb[u] * rb[u];
+			}
+		}
+		rr = pg_sum(rr, s_red);
+		bb = pg_sum(bb, s_red);
+		const double beta = sqrt(rr);
+		const double bn = bb > 0.0 ? sqrt(bb) : 1.0;
+		rel = beta / bn;
+		if (rel <= tol || iters >= max_iters)
+			break;
+		if (sizeof(T) == 4 && cycles > 0 && !(rel < 0.9 * rel_prev))
+		{
+			// an FP32 cycle that does not reduce the FP64 residual: undo it if it
+			// made things worse and leave the set to the FP64 solver
+			if (!(rel <= rel_prev))
+			{
+#pragma unroll
+				for (int i = 0; i < UPT; ++i)
+				{
+					const int u = tid + i * pblock;
+					if (u < nU)
+						x[u] -= corr[i];
+				}
+				rel = rel_prev;
+			}
+			stalled = 1;
+			break;
+		}
+		rel_prev = rel;
+		++cycles;
+		const double inv_beta = 1.0 / beta;
+		T vj[UPT];
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int u = tid + i * pblock;
+			vj[i] = (T)(ro[i] * inv_beta);
+			if (u < nU)
+				V[u] = vj[i];
+		}
+		if (tid == 0)
+		{
+			for (int i = 0; i <= m; ++i)
+				s_g[i] = 0.0;
+			s_g[0] = beta;
+		}
+
+		int k = 0;
+		for (int j = 0; j < m; ++j)
+		{
+			// t = P^{-1} v_j
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int u = tid + i * pblock;
+				if (u < nU)
+					s_t[u] = (P)vj[i];
+			}
+			__syncthreads();
+			ilu_lds<P>(G.L, lv, s_iu, s_t);
+			// w = M t on the own rows
+			double w[UPT];
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int u = tid + i * pblock;
+				w[i] = u < nU ? row_lds<P>(lv, u, dg[i], s_t) : 0.0;
+			}
+			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
+			double w2 = 0.0;
+			for (int pass = 0; pass < 2; ++pass)
+			{
+				for (int ii = 0; ii <= j; ++ii)
+				{
+					const T* __restrict__ Vi = V + (int64_t)ii * nU;
+					double d = 0.0;
+#pragma unroll
+					for (int i = 0; i < UPT; ++i)
+					{
+						const int u = tid + i * pblock;
+						if (u < nU)
+							d += w[i] * (double)Vi[u];
+					}
+					d = wave_sum(d);
+					if (lane_id() == 0)
+						s_part[ii * pwaves + tid / wave] = d;
+				}
+				__syncthreads();
+				if (tid <= j)
+				{
+					double tt = 0.0;
+					for (int q = 0; q < pwaves; ++q)
+						tt += s_part[tid * pwaves + q];
+					s_c[tid] = tt;
+					s_H[j * mh + tid] = (pass == 0 ? 0.0 : s_H[j * mh + tid]) + tt;
+				}
+				__syncthreads();
+				w2 = 0.0;
+				for (int ii = 0; ii <= j; ++ii)
+				{
+					const T* __restrict__ Vi = V + (int64_t)ii * nU;
+					const double cc = s_c[ii];
+#pragma unroll
+					for (int i = 0; i < UPT; ++i)
+					{
+						const int u = tid + i * pblock;
+						if (u < nU)
+							w[i] -= cc * (double)Vi[u];
+					}
+				}
+#pragma unroll
+				for (int i = 0; i < UPT; ++i)
+					w2 += w[i] * w[i];
+				w2 = pg_sum(w2, s_red);
+			}
+			if (tid == 0)
+			{
+				double* col = s_H + j * mh;
+				const double hn = sqrt(w2);
+				col[j + 1] = hn;
+				for (int i = 0; i < j; ++i)
+				{
+					const double a = col[i], c = col[i + 1];
+					col[i] = s_cs[i] * a + s_sn[i] * c;
+					col[i + 1] = -s_sn[i] * a + s_cs[i] * c;
+				}
+				const double a = col[j], c = col[j + 1];
+				const double rn = hypot(a, c);
+				const double cj = rn > 0.0 ? a / rn : 1.0, sj = rn > 0.0 ? c / rn : 0.0;
+				s_cs[j] = cj;
+				s_sn[j] = sj;
+				col[j] = rn;
+				col[j + 1] = 0.0;
+				s_g[j + 1] = -sj * s_g[j];
+				s_g[j] = cj * s_g[j];
+				s_scalar[1] = fabs(s_g[j + 1]) / bn;
+				s_scalar[2] = (hn > 0.0 && rn > 0.0) ? 1.0 / hn : 0.0;
+			}
+			__syncthreads();
+			const double scale = s_scalar[2];
+			T* __restrict__ Vn = V + (int64_t)(j + 1) * nU;
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int u = tid + i * pblock;
+				vj[i] = (T)(w[i] * scale);
+				if (u < nU)
+					Vn[u] = vj[i];
+			}
+			++iters;
+			k = j + 1;
+			if (s_scalar[1] <= 0.5 * tol || scale == 0.0 || iters >= max_iters)
+				break;
+		}
+		// y = H^-1 g;  x += P^{-1} (sum_i y_i V_i)
+		if (tid == 0)
+			for (int i = k - 1; i >= 0; --i)
+			{
+				double tt = s_g[i];
+				for (int l = i + 1; l < k; ++l)
+					tt -= s_H[l * mh + i] * s_c[l];
+				const double d = s_H[i * mh + i];
+				s_c[i] = d != 0.0 ? tt / d : 0.0;
+			}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int u = tid + i * pblock;
+			if (u < nU)
+			{
+				double acc = 0.0;
+				for (int ii = 0; ii < k; ++ii)
+					acc += s_c[ii] * (double)V[(int64_t)ii * nU + u];
+				s_t[u] = (P)acc;
+			}
+		}
+		__syncthreads();
+		ilu_lds<P>(G.L, lv, s_iu, s_t);
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int u = tid + i * pblock;
+			if (u < nU)
+			{
+				corr[i] = (double)s_t[u];
+				x[u] += corr[i];
+			}
+		}
+		__threadfence_block();
+		__syncthreads();
+	}
+#pragma unroll
+	for (int i = 0; i < UPT; ++i)
+	{
+		const int u = tid + i * pblock;
+		if (u < nU)
+			x[u] /= ib[u];
+	}
+	if (tid == 0)
+	{
+		out.iters[b] = iters;
+		out.cycles[b] = cycles;
+		out.res[b] = rel;
+		out.stalled[b] = stalled;
+	}
+}
+
 
 // ============================================================================
 // device-wide GMRES kernels (nbr_t coupling lists, iterate x unpreconditioned)
@@ -1884,6 +2335,49 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const nbr_t G = nb.view();
 	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get(), sc.stl.get() };
 	const double by = gmres_set_bytes(S, nb.nnz);
+	if (ilu && S.nU <= lds_max_unknowns && nb.nnz < 65536)
+	{
+		// LDS-resident kernel when its structure, pivots and vector fit
+		const int psize = o.precision == 1 ? 4 : 8;
+		const lds_plan_t lp = lds_plan(S.nU, nb.nnz, m, psize);
+		if (lp.bytes + lds_static_bytes <= std::min(c.lds_per_block, 160 * 1024))
+		{
+			const bool small = S.nU <= 4 * pblock;
+			if (lp.bytes > 48 * 1024)
+			{
+				const void* fns[4] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
+									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8> };
+				for (const void* f : fns)
+					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
+			}
+			if (o.precision == 1)
+			{
+				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				timed_launch(c, "k_gmres_lds<float>", by, [&] {
+					if (small)
+						k_gmres_lds<float, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																				 o.max_iters, po, lp);
+					else
+						k_gmres_lds<float, 8><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																				 o.max_iters, po, lp);
+				});
+			}
+			else
+			{
+				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vd.get() };
+				timed_launch(c, "k_gmres_lds<double>", by, [&] {
+					if (small)
+						k_gmres_lds<double, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																				  o.max_iters, po, lp);
+					else
+						k_gmres_lds<double, 8><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																				  o.max_iters, po, lp);
+				});
+			}
+			ESL_HIP(hipGetLastError());
+			return;
+		}
+	}
 	if (o.precision == 1)
 	{
 		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };

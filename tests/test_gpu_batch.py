@@ -133,3 +133,26 @@ def test_device_wide_solver_matches_per_set(ctx, pre):
     assert np.abs(d1.download()[0] - d16.download()[3]).sum() <= 1e-11
     if pre == 2:
         assert rep["iterations"] < 500
+
+
+@pytest.mark.parametrize("cfg,seed", [("c2", 11), ("c0", 2)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lds_resident_solver_matches_oracle(ctx, cfg, seed, precision):
+    # c2 seed 11: one terminal SCC of 2048 states (8 unknowns per thread);
+    # c0 seed 2: small systems (4 per thread); both run the LDS-resident kernel
+    B = 3
+    net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+    n = 1 << net.N
+    om = read_bnet_text(net.bnet())
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=precision, tol=1e-13)
+    xb = dout.download()
+    for k in range(B):
+        ip, rows, _ = P.transition_table(om, rates[k])
+        gr = P.transition_graph(n, ip, rows)
+        ref = P.solve_final(n, ip, rows, rates[k], x0, gr)
+        assert np.abs(xb[k] - ref).sum() <= 1e-10, (k, np.abs(xb[k] - ref).sum())
+    assert rep["residual"] <= 1e-13
