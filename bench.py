@@ -48,7 +48,52 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--graph", type=int, default=1, help="replay the numeric sequence as a HIP graph")
+    ap.add_argument("--inner-rtol", type=float, default=None,
+                    help="residual reduction an FP32 cycle aims for (library default when omitted)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the run, copy 1 GiB device to device three times (a kernel of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def traffic_record(kernel, workload):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by the same counters on a 1 GiB copy), when a
+    record for this kernel and workload exists under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            recs = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in recs:
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r
+    return None
+
+
+def copy_calibration(L, ctx):
+    """Three 1 GiB device-to-device copies through the HIP runtime's copy kernel."""
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    nbytes = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    from paraexastolog_amd._lib import check
+
+    check(L.esl_device_alloc(ctx.handle, nbytes, C.byref(src)), ctx.handle)
+    check(L.esl_device_alloc(ctx.handle, nbytes, C.byref(dst)), ctx.handle)
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    L.esl_device_free(ctx.handle, src)
+    L.esl_device_free(ctx.handle, dst)
+    print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), file=sys.stderr)
 
 
 def main():
@@ -115,7 +160,7 @@ def main():
     check(L.esl_plan_query(pl, info.ctypes.data_as(C.POINTER(C.c_int64))))
 
     def solve_batch(count, d_r, d_out, precision, tol, graph=0):
-        o = solve_opts(precision=precision, tol=tol)
+        o = solve_opts(precision=precision, tol=tol, inner_rtol=a.inner_rtol)
         o.graph = graph
         rep = SolveReport()
         check(L.esl_plan_solve(ctx.handle, pl, count, d_r, d_out, C.byref(o), C.byref(rep)), ctx.handle)
@@ -263,7 +308,14 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "traffic": None},
         "cpu_baseline": cpu,
     }
+    tr = traffic_record(dom[0], f"c1 seed {a.seed} sets {B}")
+    if tr is not None:
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = tr["source"]
     print(json.dumps(out))
+    sys.stdout.flush()
+    if a.calibrate:
+        copy_calibration(L, ctx)
     if max(l1) > 1e-10:
         print(f"bench.py: accuracy bar missed: L1 {max(l1):.3e} > 1e-10", file=sys.stderr)
         sys.exit(1)

@@ -140,6 +140,8 @@ typedef struct
 	const double* warm_pi; /* optional warm start of the terminal kernels: predicted final-vector values
 							* on the terminal states in canonical order (reordered_vertices[0, n_t)),
 							* [n_sets][n_t]; host memory for esl_solve, device memory otherwise */
+	double inner_rtol;	 /* precision 1: residual reduction one FP32 cycle aims for before the FP64
+						  * refinement step (default 1e-6; 0 keeps iterating towards tol) */
 } esl_solve_opts;
 
 typedef struct

@@ -32,6 +32,7 @@ class SolveOpts(C.Structure):
         ("preconditioner", C.c_int),
         ("graph", C.c_int),
         ("warm_pi", C.POINTER(C.c_double)),
+        ("inner_rtol", C.c_double),
     ]
 
 

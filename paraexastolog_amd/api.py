@@ -269,11 +269,13 @@ class sparse_cs_matrix:
         return len(self.indptr) - 1
 
 
-def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=32, full=0, preconditioner=0):
+def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=32, full=0, preconditioner=0, inner_rtol=None):
     o = SolveOpts()
     lib().esl_solve_opts_default(C.byref(o))
     o.precision, o.tol, o.max_iters, o.restart, o.full = precision, tol, max_iters, restart, full
     o.preconditioner = preconditioner
+    if inner_rtol is not None:
+        o.inner_rtol = inner_rtol
     return o
 
 

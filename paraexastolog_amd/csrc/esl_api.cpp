@@ -402,6 +402,7 @@ void esl_solve_opts_default(esl_solve_opts* o)
 	o->preconditioner = 0;
 	o->graph = 0;
 	o->warm_pi = nullptr;
+	o->inner_rtol = 1e-6;
 }
 
 esl_status esl_solve_batch_device(esl_ctx* ctx, const esl_graph* g, int n_sets, const double* d_rates,

@@ -1029,7 +1029,7 @@ __device__ void ilu_apply(const sys_t& S, const nbr_t& G, int n_seg, int b, cons
 template <typename T, bool ILU>
 __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
 													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
-													  int max_iters, pgm_out_t out)
+													  int max_iters, pgm_out_t out, double inner_rtol)
 {
 	__shared__ double s_H[(pm_max + 1) * pm_max];
 	__shared__ double s_cs[pm_max], s_sn[pm_max], s_g[pm_max + 1], s_c[pm_max + 1];
@@ -1116,6 +1116,9 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 			break;
 		}
 		rel_prev = rel;
+		// an FP32 cycle stops once its estimate has dropped by inner_rtol (what
+		// FP32 arithmetic can deliver); the FP64 residual decides what follows
+		const double inner_target = fmax(0.5 * tol, sizeof(T) == 4 ? inner_rtol * rel : 0.0);
 		++cycles;
 		const double inv_beta = 1.0 / beta;
 		for (int64_t u = threadIdx.x; u < nU; u += pblock)
@@ -1224,7 +1227,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 			__syncthreads();
 			++iters;
 			k = j + 1;
-			if (s_scalar[1] <= 0.5 * tol || scale == 0.0 || iters >= max_iters)
+			if (s_scalar[1] <= inner_target || scale == 0.0 || iters >= max_iters)
 				break;
 		}
 		// y = H^-1 g;  x += P^{-1} (sum_i y_i V_i)   (FP64)
@@ -1388,10 +1391,101 @@ __device__ __forceinline__ double row_lds(const lds_view_t& v, int u, double d, 
 	return acc;
 }
 
+constexpr int dot_chunk = 8; // basis vectors per batch of dot products / updates
+
+// Partial dots of w (the thread's own rows) with V_0..V_j, reduced over each
+// wavefront: s_part[ii * pwaves + wave] = sum over the wave's rows.  Eight
+// basis vectors at a time: their loads are issued together (indices clamped to
+// j, so no branches) and the eight values are reduced across the wave by a
+// transposed butterfly (halve the values per lane at distances 32, 16, 8, then
+// three plain steps): 10 shuffles per eight dots instead of 48.
+template <typename T, int UPT>
+__device__ __forceinline__ void wave_dots(const T* __restrict__ V, int nU, int j, const double (&w)[UPT],
+										  double* __restrict__ s_part)
+{
+	const int tid = threadIdx.x, lane = tid & (wave - 1), wv = tid / wave;
+	for (int c0 = 0; c0 <= j; c0 += dot_chunk)
+	{
+		double v[dot_chunk];
+#pragma unroll
+		for (int q = 0; q < dot_chunk; ++q)
+		{
+			const int ii = c0 + q;
+			const T* __restrict__ Vi = V + (int64_t)(ii <= j ? ii : j) * nU;
+			double d = 0.0;
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int u = tid + i * pblock;
+				if (u < nU)
+					d += w[i] * (double)Vi[u];
+			}
+			v[q] = ii <= j ? d : 0.0;
+		}
+		{
+			const bool hi = lane & 32;
+#pragma unroll
+			for (int k = 0; k < 4; ++k)
+			{
+				const double send = hi ? v[k] : v[k + 4];
+				const double keep = hi ? v[k + 4] : v[k];
+				v[k] = keep + __shfl_xor(send, 32, wave);
+			}
+		}
+		{
+			const bool hi = lane & 16;
+#pragma unroll
+			for (int k = 0; k < 2; ++k)
+			{
+				const double send = hi ? v[k] : v[k + 2];
+				const double keep = hi ? v[k + 2] : v[k];
+				v[k] = keep + __shfl_xor(send, 16, wave);
+			}
+		}
+		{
+			const bool hi = lane & 8;
+			const double send = hi ? v[0] : v[1];
+			const double keep = hi ? v[1] : v[0];
+			v[0] = keep + __shfl_xor(send, 8, wave);
+		}
+		v[0] += __shfl_xor(v[0], 4, wave);
+		v[0] += __shfl_xor(v[0], 2, wave);
+		v[0] += __shfl_xor(v[0], 1, wave);
+		const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+		if ((lane & 7) == 0 && c0 + idx <= j)
+			s_part[(c0 + idx) * pwaves + wv] = v[0];
+	}
+}
+
+// w -= sum_ii c[ii] V_ii over ii = 0..j on the own rows (loads batched as above)
+template <typename T, int UPT>
+__device__ __forceinline__ void wave_update(const T* __restrict__ V, int nU, int j, const double* __restrict__ c,
+											double (&w)[UPT])
+{
+	const int tid = threadIdx.x;
+	for (int c0 = 0; c0 <= j; c0 += dot_chunk)
+	{
+#pragma unroll
+		for (int q = 0; q < dot_chunk; ++q)
+		{
+			const int ii = c0 + q;
+			const T* __restrict__ Vi = V + (int64_t)(ii <= j ? ii : j) * nU;
+			const double cc = ii <= j ? c[ii] : 0.0;
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int u = tid + i * pblock;
+				if (u < nU)
+					w[i] -= cc * (double)Vi[u];
+			}
+		}
+	}
+}
+
 template <typename T, int UPT>
 __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
 													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
-													  int max_iters, pgm_out_t out, lds_plan_t lp)
+													  int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
 {
 	using P = T; // precision of the LDS vectors (preconditioned vector, inverse pivots)
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -1538,6 +1632,9 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			break;
 		}
 		rel_prev = rel;
+		// an FP32 cycle stops once its estimate has dropped by inner_rtol (what
+		// FP32 arithmetic can deliver); the FP64 residual decides what follows
+		const double inner_target = fmax(0.5 * tol, sizeof(T) == 4 ? inner_rtol * rel : 0.0);
 		++cycles;
 		const double inv_beta = 1.0 / beta;
 		T vj[UPT];
@@ -1581,21 +1678,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			double w2 = 0.0;
 			for (int pass = 0; pass < 2; ++pass)
 			{
-				for (int ii = 0; ii <= j; ++ii)
-				{
-					const T* __restrict__ Vi = V + (int64_t)ii * nU;
-					double d = 0.0;
-#pragma unroll
-					for (int i = 0; i < UPT; ++i)
-					{
-						const int u = tid + i * pblock;
-						if (u < nU)
-							d += w[i] * (double)Vi[u];
-					}
-					d = wave_sum(d);
-					if (lane_id() == 0)
-						s_part[ii * pwaves + tid / wave] = d;
-				}
+				wave_dots<T, UPT>(V, nU, j, w, s_part);
 				__syncthreads();
 				if (tid <= j)
 				{
@@ -1607,18 +1690,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 				}
 				__syncthreads();
 				w2 = 0.0;
-				for (int ii = 0; ii <= j; ++ii)
-				{
-					const T* __restrict__ Vi = V + (int64_t)ii * nU;
-					const double cc = s_c[ii];
-#pragma unroll
-					for (int i = 0; i < UPT; ++i)
-					{
-						const int u = tid + i * pblock;
-						if (u < nU)
-							w[i] -= cc * (double)Vi[u];
-					}
-				}
+				wave_update<T, UPT>(V, nU, j, s_c, w);
 #pragma unroll
 				for (int i = 0; i < UPT; ++i)
 					w2 += w[i] * w[i];
@@ -1660,7 +1732,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			}
 			++iters;
 			k = j + 1;
-			if (s_scalar[1] <= 0.5 * tol || scale == 0.0 || iters >= max_iters)
+			if (s_scalar[1] <= inner_target || scale == 0.0 || iters >= max_iters)
 				break;
 		}
 		// y = H^-1 g;  x += P^{-1} (sum_i y_i V_i)
@@ -2160,7 +2232,7 @@ struct gmres_t
 	// x: FP64 iterate (in/out).  Returns false when an FP32 cycle failed to
 	// reduce the FP64 residual (the cycle is undone if it increased it); the
 	// caller then continues in FP64 from x.
-	bool solve(const double* rhs, double* x, double tol, int max_iters)
+	bool solve(const double* rhs, double* x, double tol, int max_iters, double inner_rtol = 0.0)
 	{
 		double prev = 0.0;
 		const int64_t nU = S.nU;
@@ -2197,6 +2269,7 @@ struct gmres_t
 				return false;
 			}
 			prev = final_res;
+			const double inner_target = std::max(0.5 * tol, sizeof(T) == 4 ? inner_rtol * final_res : 0.0);
 			if (iters >= max_iters)
 				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
 											   std::to_string(final_res) + " after " + std::to_string(iters) +
@@ -2233,7 +2306,7 @@ struct gmres_t
 				++iters;
 				k = j + 1;
 				const double est = max_res();
-				if (est <= 0.5 * tol || iters >= max_iters)
+				if (est <= inner_target || iters >= max_iters)
 					break;
 			}
 			// x += P^{-1} (sum_i y_i V_i) in FP64
@@ -2356,10 +2429,10 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 				timed_launch(c, "k_gmres_lds<float>", by, [&] {
 					if (small)
 						k_gmres_lds<float, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																				 o.max_iters, po, lp);
+																				 o.max_iters, po, lp, o.inner_rtol);
 					else
 						k_gmres_lds<float, 8><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																				 o.max_iters, po, lp);
+																				 o.max_iters, po, lp, o.inner_rtol);
 				});
 			}
 			else
@@ -2368,10 +2441,10 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 				timed_launch(c, "k_gmres_lds<double>", by, [&] {
 					if (small)
 						k_gmres_lds<double, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																				  o.max_iters, po, lp);
+																				  o.max_iters, po, lp, o.inner_rtol);
 					else
 						k_gmres_lds<double, 8><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																				  o.max_iters, po, lp);
+																				  o.max_iters, po, lp, o.inner_rtol);
 				});
 			}
 			ESL_HIP(hipGetLastError());
@@ -2384,10 +2457,10 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 		timed_launch(c, ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<float, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
-																	 po);
+																	 po, o.inner_rtol);
 			else
 				k_gmres_set<float, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																	  o.max_iters, po);
+																	  o.max_iters, po, o.inner_rtol);
 		});
 	}
 	else
@@ -2396,10 +2469,10 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 		timed_launch(c, ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<double, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
-																	  po);
+																	  po, o.inner_rtol);
 			else
 				k_gmres_set<double, false><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																	   o.max_iters, po);
+																	   o.max_iters, po, o.inner_rtol);
 		});
 	}
 	ESL_HIP(hipGetLastError());
@@ -2458,12 +2531,13 @@ struct plan_t
 		double* fin = nullptr;
 		const double* warm = nullptr;
 		int precision = 0, preconditioner = 0, m = 0;
-		double tol = 0;
+		double tol = 0, inner_rtol = 0;
 		int max_iters = 0;
 		bool operator==(const gkey_t& o) const
 		{
 			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
-				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && max_iters == o.max_iters;
+				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && inner_rtol == o.inner_rtol &&
+				   max_iters == o.max_iters;
 		}
 	};
 	gkey_t gkey{};
@@ -2528,7 +2602,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		if (opts.precision == 1)
 		{
 			gmres_t<float> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			done = gm.solve(rhs_p, z_p, tol, max_iters);
+			done = gm.solve(rhs_p, z_p, tol, max_iters, opts.inner_rtol);
 			kr.resid = gm.final_res;
 			kr.iters = gm.iters;
 			kr.cycles = gm.cycles;
@@ -2849,7 +2923,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		if (use_graph)
 		{
 			const plan_t::gkey_t key{ B, d_rates, d_final, o.warm_pi, o.precision, o.preconditioner, m, o.tol,
-									  o.max_iters };
+									  o.inner_rtol, o.max_iters };
 			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
 			{
 				P.drop_graph();
