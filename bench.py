@@ -73,24 +73,16 @@ def traffic_record(kernel, workload):
 
 
 def copy_calibration(L, ctx):
-    """Three 1 GiB device-to-device copies through the HIP runtime's copy kernel."""
-    try:
-        hip = C.CDLL("libamdhip64.so")
-    except OSError:
-        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
-    nbytes = 1 << 30
-    src, dst = C.c_void_p(), C.c_void_p()
+    """Three 1 GiB device-to-device copies by the library's copy kernel (each
+    reads and writes 1 GiB once)."""
     from paraexastolog_amd._lib import check
 
+    nbytes = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
     check(L.esl_device_alloc(ctx.handle, nbytes, C.byref(src)), ctx.handle)
     check(L.esl_device_alloc(ctx.handle, nbytes, C.byref(dst)), ctx.handle)
-    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    hip.hipDeviceSynchronize.argtypes = []
     for _ in range(3):
-        if hip.hipMemcpy(dst, src, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
-            raise RuntimeError("hipMemcpy failed")
-    if hip.hipDeviceSynchronize() != 0:
-        raise RuntimeError("hipDeviceSynchronize failed")
+        check(L.esl_memcpy_d2d(ctx.handle, dst, src, nbytes), ctx.handle)
     L.esl_device_free(ctx.handle, src)
     L.esl_device_free(ctx.handle, dst)
     print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), file=sys.stderr)
@@ -308,7 +300,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "traffic": None},
         "cpu_baseline": cpu,
     }
-    tr = traffic_record(dom[0], f"c1 seed {a.seed} sets {B}")
+    tr = traffic_record(dom[0], f"N=20 seed {a.seed} sets {B}")
     if tr is not None:
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
         out["roofline"]["traffic_source"] = tr["source"]

@@ -216,6 +216,9 @@ esl_status esl_device_alloc(esl_ctx* ctx, size_t bytes, void** out);
 esl_status esl_device_free(esl_ctx* ctx, void* p);
 esl_status esl_memcpy_h2d(esl_ctx* ctx, void* dst, const void* src, size_t bytes);
 esl_status esl_memcpy_d2h(esl_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* device to device on the ctx stream, by the library's own copy kernel (reads and
+ * writes `bytes` once each: also a kernel of known traffic for counter scaling) */
+esl_status esl_memcpy_d2d(esl_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- kernel profiling (HIP events on the ctx stream) -----------------------
  * enable != 0 starts collecting per-kernel launch counts, device time and

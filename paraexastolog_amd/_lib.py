@@ -98,6 +98,7 @@ SIGNATURES = [
     ("esl_device_free", C.c_int, [_VP, _VP]),
     ("esl_memcpy_h2d", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_memcpy_d2h", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+    ("esl_memcpy_d2d", C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     ("esl_graph_stage_ms", C.c_int, [_VP, _P(C.c_double)]),
     ("esl_profile", C.c_int, [_VP, C.c_int]),
     ("esl_plan_create", C.c_int, [_VP, _VP, _VP, _P(_VP)]),

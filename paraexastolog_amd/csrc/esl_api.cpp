@@ -630,4 +630,14 @@ esl_status esl_memcpy_d2h(esl_ctx* ctx, void* dst, const void* src, size_t bytes
 	});
 }
 
+esl_status esl_memcpy_d2d(esl_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+	return guarded(ctx, [&] {
+		require(ctx && dst && src, ESL_E_ARG, "null argument");
+		set_device(ctx);
+		esl::device_copy(ctx->c, dst, src, bytes);
+		ESL_HIP(hipStreamSynchronize(ctx->c.stream));
+	});
+}
+
 } // extern "C"

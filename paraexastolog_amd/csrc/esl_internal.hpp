@@ -258,6 +258,7 @@ struct graph_t
 
 // ---- kernel launchers (defined in the .hip files) ----
 void table_build(ctx_t& c, graph_t& g, const model_t& m);
+void device_copy(ctx_t& c, void* dst, const void* src, size_t bytes);
 void graph_reorganize(ctx_t& c, graph_t& g);
 
 struct solve_args_t

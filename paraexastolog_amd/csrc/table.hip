@@ -259,4 +259,31 @@ void table_build(ctx_t& c, graph_t& g, const model_t& m)
 	g.stage_ms[0] = ms;
 }
 
+// ---- device-to-device copy: 16-byte vectors when both ends allow it, bytes otherwise
+__global__ __launch_bounds__(256) void k_copy16(int64_t count, const uint4* __restrict__ src, uint4* __restrict__ dst)
+{
+	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
+		dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy1(int64_t count, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst)
+{
+	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
+		dst[i] = src[i];
+}
+
+void device_copy(ctx_t& c, void* dst, const void* src, size_t bytes)
+{
+	if (!bytes)
+		return;
+	const bool vec = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0) && bytes % 16 == 0;
+	const int64_t count = vec ? (int64_t)(bytes / 16) : (int64_t)bytes;
+	const int blocks = (int)std::min<int64_t>((count + 255) / 256, 256 * 64);
+	if (vec)
+		k_copy16<<<blocks, 256, 0, c.stream>>>(count, (const uint4*)src, (uint4*)dst);
+	else
+		k_copy1<<<blocks, 256, 0, c.stream>>>(count, (const uint8_t*)src, (uint8_t*)dst);
+	ESL_HIP(hipGetLastError());
+}
+
 } // namespace esl

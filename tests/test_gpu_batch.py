@@ -156,3 +156,18 @@ def test_lds_resident_solver_matches_oracle(ctx, cfg, seed, precision):
         ref = P.solve_final(n, ip, rows, rates[k], x0, gr)
         assert np.abs(xb[k] - ref).sum() <= 1e-10, (k, np.abs(xb[k] - ref).sum())
     assert rep["residual"] <= 1e-13
+
+
+def test_device_copy(ctx):
+    from paraexastolog_amd._lib import check, lib
+    import ctypes as C
+
+    rng = np.random.default_rng(5)
+    for nbytes, off in ((1 << 20, 0), (1000003, 0), (4096 + 7, 3)):
+        src = rng.integers(0, 256, nbytes + off, dtype=np.uint8)
+        ds = DeviceArray.from_host(ctx, src)
+        dd = DeviceArray(ctx, (nbytes + off,), np.uint8)
+        check(lib().esl_memcpy_d2d(ctx.handle, C.c_void_p(dd.ptr.value + off), C.c_void_p(ds.ptr.value + off),
+                                   nbytes), ctx.handle)
+        got = dd.download()
+        np.testing.assert_array_equal(got[off:], src[off:])
