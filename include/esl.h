@@ -211,6 +211,17 @@ esl_status esl_plan_query(const esl_plan* p, int64_t info[8]);
 esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 						  const esl_solve_opts* opts, esl_solve_report* report);
 
+/* Several plans (distinct networks, e.g. of uneven size) in one call: plan i solves
+ * n_sets[i] rate sets d_rates[i] -> d_final[i]; the per-set solves of all plans run
+ * concurrently on forked streams (opts->graph = 1: captured once as one HIP graph and
+ * replayed while plans, sizes, pointers and options stay the same).  Plans whose system
+ * needs the device-wide solver run one after another first.  reports[n_plans].
+ * No reference counterpart (the reference solves one network per process); it serves
+ * batches over several networks. */
+esl_status esl_plan_solve_many(esl_ctx* ctx, int n_plans, esl_plan* const* plans, const int* n_sets,
+							   const double* const* d_rates, double* const* d_final, const esl_solve_opts* opts,
+							   esl_solve_report* reports);
+
 /* ---- device memory helpers (for callers without their own allocator) ------- */
 esl_status esl_device_alloc(esl_ctx* ctx, size_t bytes, void** out);
 esl_status esl_device_free(esl_ctx* ctx, void* p);

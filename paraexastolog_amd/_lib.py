@@ -105,6 +105,8 @@ SIGNATURES = [
     ("esl_plan_destroy", None, [_VP]),
     ("esl_plan_query", C.c_int, [_VP, _P(C.c_int64)]),
     ("esl_plan_solve", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
+    ("esl_plan_solve_many", C.c_int, [_VP, C.c_int, _P(_VP), _P(C.c_int), _P(_VP), _P(_VP), _P(SolveOpts),
+                                      _P(SolveReport)]),
     ("esl_mlp_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _P(_VP)]),
     ("esl_mlp_destroy", None, [_VP]),
     ("esl_mlp_num_params", C.c_int64, [_VP]),

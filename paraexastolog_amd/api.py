@@ -385,6 +385,20 @@ class solve_plan:
                                    C.byref(rep)), self.context.handle)
         return rep.as_dict()
 
+    @staticmethod
+    def solve_many(context, plans, n_sets, d_rates, d_finals, **opts):
+        """Several plans in one call (include/esl.h esl_plan_solve_many): the
+        per-set solves of all plans run concurrently.  Returns one report per plan."""
+        k = len(plans)
+        o = solve_opts(**{key: v for key, v in opts.items() if key != "graph"})
+        o.graph = int(opts.get("graph", 0))
+        reps = (SolveReport * k)()
+        arr = lambda vals: (C.c_void_p * k)(*[C.c_void_p(getattr(v, "value", v)) for v in vals])  # noqa: E731
+        check(lib().esl_plan_solve_many(context.handle, k, (C.c_void_p * k)(*[p.handle for p in plans]),
+                                        (C.c_int * k)(*n_sets), arr(d_rates), arr(d_finals), C.byref(o), reps),
+              context.handle)
+        return [r.as_dict() for r in reps]
+
     def close(self):
         if self.handle:
             lib().esl_plan_destroy(self.handle)

@@ -143,6 +143,15 @@ void esl_ctx_destroy(esl_ctx* ctx)
 	ctx->c.flags.reset();
 	ctx->c.scratch_a.reset();
 	ctx->c.scratch_b.reset();
+	if (ctx->c.multi && ctx->c.multi_free)
+		ctx->c.multi_free(ctx->c.multi);
+	for (auto s : ctx->c.aux)
+	{
+		(void)hipStreamSynchronize(s);
+		(void)hipStreamDestroy(s);
+	}
+	for (auto e : ctx->c.aux_ev)
+		(void)hipEventDestroy(e);
 	if (ctx->c.stream)
 		(void)hipStreamDestroy(ctx->c.stream);
 	delete ctx;
@@ -586,6 +595,35 @@ esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d
 		esl::plan_solve(ctx->c, p->p, n_sets, d_rates, d_final, o, rep);
 		if (report)
 			*report = rep;
+	});
+}
+
+esl_status esl_plan_solve_many(esl_ctx* ctx, int n_plans, esl_plan* const* plans, const int* n_sets,
+							   const double* const* d_rates, double* const* d_final, const esl_solve_opts* opts,
+							   esl_solve_report* reports)
+{
+	return guarded(ctx, [&] {
+		require(ctx && plans && n_sets && d_rates && d_final && reports, ESL_E_ARG, "null argument");
+		require(n_plans >= 1, ESL_E_ARG, "n_plans must be >= 1");
+		std::vector<void*> ps(n_plans);
+		for (int i = 0; i < n_plans; ++i)
+		{
+			require(plans[i] && d_rates[i] && d_final[i], ESL_E_ARG, "null plan or buffer");
+			require(plans[i]->ctx == ctx, ESL_E_ARG, "plan belongs to another context");
+			require(n_sets[i] >= 1, ESL_E_ARG, "n_sets must be >= 1");
+			for (int j = 0; j < i; ++j)
+				require(plans[j] != plans[i], ESL_E_ARG, "a plan appears twice (its buffers would be shared)");
+			ps[i] = plans[i]->p;
+		}
+		set_device(ctx);
+		esl_solve_opts o;
+		esl_solve_opts_default(&o);
+		if (opts)
+			o = *opts;
+		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		require(!o.full, ESL_E_ARG, "opts.full is not available on plans; use esl_solve");
+		require(o.warm_pi == nullptr, ESL_E_ARG, "warm starts are per plan: use esl_plan_solve");
+		esl::plan_solve_many(ctx->c, n_plans, ps.data(), n_sets, d_rates, d_final, o, reports);
 	});
 }
 

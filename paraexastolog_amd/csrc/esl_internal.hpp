@@ -165,6 +165,12 @@ struct ctx_t
 	dbuf<int32_t> flags;	// small device counters / flags
 	int32_t* h_flags = nullptr; // pinned mirror
 	prof_t prof;
+	// forked streams (default priority) and fork/join events for plans solved concurrently
+	std::vector<hipStream_t> aux;
+	std::vector<hipEvent_t> aux_ev;
+	// captured sequence of the last concurrent multi-plan solve (owned by solve.hip)
+	void* multi = nullptr;
+	void (*multi_free)(void*) = nullptr;
 };
 
 // Appends an event-record node to the graph being captured on `s`, after the
@@ -276,6 +282,8 @@ void plan_destroy(void* p);
 void plan_solve(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
 				esl_solve_report& rep);
 void plan_info(const void* p, int64_t info[8]);
+void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const double* const* d_rates,
+					 double* const* d_final, const esl_solve_opts& o, esl_solve_report* reps);
 // the implementation object behind an esl_ctx handle (defined in esl_api.cpp)
 ctx_t& ctx_impl(esl_ctx* ctx);
 // solution_term / solution_nonterm of one rate set into g.sol (reference layout)

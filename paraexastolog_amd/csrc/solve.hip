@@ -2522,6 +2522,7 @@ struct plan_t
 	dbuf<int32_t> anchor;
 	per_set_scratch_t scr;
 	int anchor_sweeps = 32;
+	int64_t epoch = 0; // bumped whenever a per-run buffer is reallocated
 	double ms[3] = { 0, 0, 0 };
 	// captured numeric sequence and the arguments it was captured with
 	struct gkey_t
@@ -2533,11 +2534,12 @@ struct plan_t
 		int precision = 0, preconditioner = 0, m = 0;
 		double tol = 0, inner_rtol = 0;
 		int max_iters = 0;
+		int64_t epoch = 0;
 		bool operator==(const gkey_t& o) const
 		{
 			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
 				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && inner_rtol == o.inner_rtol &&
-				   max_iters == o.max_iters;
+				   max_iters == o.max_iters && epoch == o.epoch;
 		}
 	};
 	gkey_t gkey{};
@@ -2886,7 +2888,26 @@ void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_f
 	plan_enqueue_post(c, P, B, d_rates, d_final);
 }
 
+void plan_reserve_(plan_t& P, int B, int m, int precision);
+
+// grows the per-run buffers; a captured graph holds their addresses, so any
+// reallocation bumps the plan's epoch (part of every graph key)
 void plan_reserve(plan_t& P, int B, int m, int precision)
+{
+	auto ptrs = [&] {
+		return std::vector<const void*>{ P.p0.get(),	 P.p1.get(),	  P.anchor.get(), P.invd.get(),
+										 P.rhs.get(),	 P.z.get(),		  P.y.get(),	  P.segsum.get(),
+										 P.fpart.get(), P.scr.r.get(),	  P.scr.udiag.get(), P.scr.res.get(),
+										 P.scr.t.get(), P.scr.its.get(), P.scr.cyc.get(), P.scr.stl.get(),
+										 P.scr.Vf.get(), P.scr.Vd.get() };
+	};
+	const auto before = ptrs();
+	plan_reserve_(P, B, m, precision);
+	if (ptrs() != before)
+		++P.epoch;
+}
+
+void plan_reserve_(plan_t& P, int B, int m, int precision)
 {
 	const int64_t nU = std::max<int64_t>(P.nU, 1), nUt = std::max<int64_t>(P.nUt, 1);
 	P.p0.ensure((size_t)B * nUt);
@@ -2922,8 +2943,8 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		const bool use_graph = o.graph != 0;
 		if (use_graph)
 		{
-			const plan_t::gkey_t key{ B, d_rates, d_final, o.warm_pi, o.precision, o.preconditioner, m, o.tol,
-									  o.inner_rtol, o.max_iters };
+			const plan_t::gkey_t key{ B,   d_rates,		 d_final,	  o.warm_pi, o.precision, o.preconditioner,
+									  m,   o.tol,		 o.inner_rtol, o.max_iters, P.epoch };
 			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
 			{
 				P.drop_graph();
@@ -3017,6 +3038,201 @@ void plan_solve(ctx_t& c, void* p, int B, const double* d_rates, double* d_final
 				esl_solve_report& rep)
 {
 	plan_run(c, *static_cast<plan_t*>(p), B, d_rates, d_final, o, rep);
+}
+
+// ---- several plans (distinct networks) in one call: their per-set solves run
+// concurrently on forked streams, optionally captured as one HIP graph
+struct multi_graph_t
+{
+	std::vector<void*> plans;
+	std::vector<int> B;
+	std::vector<const double*> rates;
+	std::vector<double*> fin;
+	std::vector<int64_t> epochs;
+	int precision = 0, preconditioner = 0, m = 0, max_iters = 0;
+	double tol = 0, inner_rtol = 0;
+	bool prof = false;
+	hipGraphExec_t exec = nullptr;
+	std::vector<hipEvent_t> gev;
+	std::vector<prof_t::pending_t> gpend;
+	bool same(const multi_graph_t& o) const
+	{
+		return plans == o.plans && B == o.B && rates == o.rates && fin == o.fin && epochs == o.epochs &&
+			   precision == o.precision &&
+			   preconditioner == o.preconditioner && m == o.m && max_iters == o.max_iters && tol == o.tol &&
+			   inner_rtol == o.inner_rtol && prof == o.prof;
+	}
+	~multi_graph_t()
+	{
+		if (exec)
+			(void)hipGraphExecDestroy(exec);
+		for (auto e : gev)
+			(void)hipEventDestroy(e);
+	}
+};
+
+void multi_graph_free(void* p) { delete static_cast<multi_graph_t*>(p); }
+
+void ensure_aux(ctx_t& c, int n)
+{
+	while ((int)c.aux.size() < n)
+	{
+		hipStream_t s;
+		ESL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		c.aux.push_back(s);
+	}
+	while ((int)c.aux_ev.size() < n + 1)
+	{
+		hipEvent_t e;
+		ESL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		c.aux_ev.push_back(e);
+	}
+}
+
+void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const double* const* d_rates,
+					 double* const* d_final, const esl_solve_opts& o_in, esl_solve_report* reps)
+{
+	esl_solve_opts o = o_in;
+	if (o.tol <= 0)
+		o.tol = 1e-13;
+	if (o.max_iters <= 0)
+		o.max_iters = 20000;
+	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
+	std::vector<int> conc;
+	for (int i = 0; i < np; ++i)
+	{
+		plan_t& P = *static_cast<plan_t*>(plans[i]);
+		reps[i] = esl_solve_report{};
+		if (P.nU == 0 || P.nU <= per_set_max_unknowns(B[i]))
+			conc.push_back(i);
+		else // one large system: its device-wide solver is driven from the host
+			plan_run(c, P, B[i], d_rates[i], d_final[i], o, reps[i]);
+	}
+	if (conc.empty())
+		return;
+	for (int i : conc)
+		plan_reserve(*static_cast<plan_t*>(plans[i]), B[i], m, o.precision);
+	ensure_aux(c, (int)conc.size());
+	timer_t tm_(c);
+	auto enqueue_all = [&] {
+		const hipStream_t main = c.stream;
+		ESL_HIP(hipEventRecord(c.aux_ev[0], main));
+		for (size_t k = 0; k < conc.size(); ++k)
+		{
+			const int i = conc[k];
+			ESL_HIP(hipStreamWaitEvent(c.aux[k], c.aux_ev[0], 0));
+			c.stream = c.aux[k];
+			try
+			{
+				plan_enqueue(c, *static_cast<plan_t*>(plans[i]), B[i], d_rates[i], d_final[i], o);
+			}
+			catch (...)
+			{
+				c.stream = main;
+				throw;
+			}
+			c.stream = main;
+			ESL_HIP(hipEventRecord(c.aux_ev[k + 1], c.aux[k]));
+			ESL_HIP(hipStreamWaitEvent(main, c.aux_ev[k + 1], 0));
+		}
+	};
+	if (o.graph)
+	{
+		multi_graph_t key;
+		for (int i : conc)
+		{
+			key.plans.push_back(plans[i]);
+			key.B.push_back(B[i]);
+			key.rates.push_back(d_rates[i]);
+			key.fin.push_back(d_final[i]);
+			key.epochs.push_back(static_cast<plan_t*>(plans[i])->epoch);
+		}
+		key.precision = o.precision;
+		key.preconditioner = o.preconditioner;
+		key.m = m;
+		key.max_iters = o.max_iters;
+		key.tol = o.tol;
+		key.inner_rtol = o.inner_rtol;
+		key.prof = c.prof.on;
+		multi_graph_t* mg = static_cast<multi_graph_t*>(c.multi);
+		if (!mg || !mg->same(key))
+		{
+			if (c.multi_free && c.multi)
+				c.multi_free(c.multi);
+			c.multi = nullptr;
+			mg = new multi_graph_t(std::move(key));
+			c.multi = mg;
+			c.multi_free = multi_graph_free;
+			if (c.prof.on)
+			{
+				ESL_HIP(hipStreamSynchronize(c.stream));
+				c.prof.flush();
+				c.prof.capture_events = &mg->gev;
+				c.prof.capture_pending = &mg->gpend;
+			}
+			hipGraph_t graph;
+			ESL_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+			try
+			{
+				enqueue_all();
+			}
+			catch (...)
+			{
+				(void)hipStreamEndCapture(c.stream, &graph);
+				c.prof.capture_events = nullptr;
+				c.prof.capture_pending = nullptr;
+				c.multi_free(c.multi);
+				c.multi = nullptr;
+				throw;
+			}
+			c.prof.capture_events = nullptr;
+			c.prof.capture_pending = nullptr;
+			ESL_HIP(hipStreamEndCapture(c.stream, &graph));
+			const hipError_t ie = hipGraphInstantiate(&mg->exec, graph, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(graph);
+			if (ie != hipSuccess)
+			{
+				c.multi_free(c.multi);
+				c.multi = nullptr;
+				ESL_HIP(ie);
+			}
+		}
+		ESL_HIP(hipGraphLaunch(mg->exec, c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		if (c.prof.on)
+			c.prof.book(mg->gpend);
+	}
+	else
+		enqueue_all();
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	for (int i : conc)
+	{
+		plan_t& P = *static_cast<plan_t*>(plans[i]);
+		esl_solve_report& rep = reps[i];
+		rep.terminals_reached = P.Tr;
+		rep.unknowns = P.nU;
+		rep.reachable = -1;
+		if (P.nU > 0)
+		{
+			esl_solve_opts of = o;
+			of.restart = m;
+			const krylov_result_t kr =
+				finish_per_set(c, P.sys(B[i], d_rates[i]), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), of, P.scr);
+			if (kr.fp64_fallback)
+				plan_enqueue_post(c, P, B[i], d_rates[i], d_final[i]);
+			rep.residual = kr.resid;
+			rep.iterations = kr.iters;
+			rep.refinements = kr.cycles;
+			rep.sets_iterated = kr.sets_iterated;
+			rep.fp64_fallback = kr.fp64_fallback;
+		}
+	}
+	const double ms = tm_.lap();
+	for (int i : conc)
+	{
+		reps[i].ms_stage[3] = ms;
+		reps[i].ms_total = ms;
+	}
 }
 
 void plan_info(const void* p, int64_t info[8])

@@ -69,6 +69,7 @@ class Network:
     fixed: list = field(default_factory=list)  # node indices
     fixed_vals: list = field(default_factory=list)
     seed: int = 0
+    cycles: list = field(default_factory=list)  # spliced (a, b, c) triples
 
     @property
     def nodes(self):
@@ -133,12 +134,35 @@ def rate_sets(rng: Xoshiro256ss, N: int, count: int, lo: float = 0.1, hi: float 
     return out
 
 
+def splice_cycles(net: Network, rng: Xoshiro256ss, count: int) -> list:
+    """Splices `count` negative-feedback 3-cycles a -> b -> c -| a into the rules:
+    3*count distinct nodes are drawn by a partial Fisher-Yates shuffle (after the
+    fixed-node draws), taken three at a time as (a, b, c), and their rules are
+    replaced by f_b = a, f_c = b, f_a = !c.  Each cycle alone has no fixed point
+    and a 6-state cyclic attractor on its three bits; nodes that read the cycle
+    nodes spread it, which is what makes the terminal SCCs large.  Returns the
+    triples."""
+    N = net.N
+    pool = list(range(N))
+    for a in range(3 * count):
+        b = a + rng.below(N - a)
+        pool[a], pool[b] = pool[b], pool[a]
+    triples = [tuple(pool[3 * t:3 * t + 3]) for t in range(count)]
+    for a, b, c in triples:
+        net.inputs[b], net.tables[b] = [a], [0, 1]
+        net.inputs[c], net.tables[c] = [b], [0, 1]
+        net.inputs[a], net.tables[a] = [c], [1, 0]
+    return triples
+
+
 # The workloads the benchmarks and tests use.
 def config(name: str, seed: int = 0):
     """(network, rng) for a named workload:
     c0  N=10, K=2, nodes v0..v2 fixed to 0 (128 initial states), rates 1.0
     c1  N=20, K in {2,3}, 5 seeded fixed nodes, log-uniform rates
     c2  N=16, K=2, single topology for rate-set batches
+    c3  N=22, K=3, 5 seeded fixed nodes, then 3 spliced negative-feedback
+        3-cycles (splice_cycles), log-uniform rates
     """
     if name == "c0":
         net, rng = generate(10, 2, seed, 0)
@@ -148,4 +172,18 @@ def config(name: str, seed: int = 0):
         return generate(20, 0, seed, 5)
     if name == "c2":
         return generate(16, 2, seed, 0)
+    if name == "c3":
+        net, rng = generate(22, 3, seed, 5)
+        net.cycles = splice_cycles(net, rng, 3)
+        return net, rng
     raise KeyError(name)
+
+
+UNEVEN_SIZES = (12, 13, 14, 15, 16, 17, 18, 18)
+
+
+def uneven(seed: int = 0):
+    """Eight distinct networks of N = 12, 13, ..., 18, 18 (K = 2, no fixed
+    nodes): network k is generate(N_k, 2, 1000 * seed + k).  Returns a list of
+    (network, rng)."""
+    return [generate(n, 2, 1000 * seed + k, 0) for k, n in enumerate(UNEVEN_SIZES)]

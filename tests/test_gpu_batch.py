@@ -40,7 +40,8 @@ def test_plan_batch_matches_single_solves(ctx, cfg, seed, precision):
         s = solver(ctx, t, g, rates[k], x0)
         s.solve()
         assert np.abs(xb[k] - s.final_state).sum() <= 1e-10
-        assert abs(xb[k].sum() - 1) < 1e-12
+        # mass is conserved up to the residual target (1e-12 relative) times the conditioning
+        assert abs(xb[k].sum() - 1) < 1e-11
     assert rep["residual"] <= 1e-12
 
 
@@ -171,3 +172,32 @@ def test_device_copy(ctx):
                                    nbytes), ctx.handle)
         got = dd.download()
         np.testing.assert_array_equal(got[off:], src[off:])
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_solve_many_matches_single_plans(ctx, graph):
+    # networks of uneven size solved concurrently == each plan on its own
+    specs = [("c0", 1, 5), ("c2", 11, 3), ("c0", 4, 7), ("c2", 37, 2)]
+    plans, nsets, drs, dfs, singles, keep = [], [], [], [], [], []
+    for cfg, seed, B in specs:
+        net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+        n = 1 << net.N
+        dx0 = DeviceArray.from_host(ctx, x0)
+        dr = DeviceArray.from_host(ctx, rates)
+        df = DeviceArray(ctx, (B, n), np.float64)
+        ds = DeviceArray(ctx, (B, n), np.float64)
+        p = solve_plan(ctx, t, dx0.ptr)
+        p.solve(B, dr.ptr, ds.ptr, precision=1, tol=1e-12)
+        plans.append(p)
+        nsets.append(B)
+        drs.append(dr.ptr)
+        dfs.append(df.ptr)
+        singles.append(ds)
+        keep += [t, g, dx0, dr, df]
+    for rep_i in range(3 if graph else 1):
+        reps = solve_plan.solve_many(ctx, plans, nsets, drs, dfs, precision=1, tol=1e-12, graph=graph)
+        for k in range(len(specs)):
+            got = keep[5 * k + 4].download()
+            ref = singles[k].download()
+            assert np.abs(got - ref).sum(axis=1).max() <= 1e-10, (rep_i, k)
+            assert reps[k]["residual"] <= 1e-12
