@@ -211,6 +211,16 @@ esl_status esl_plan_query(const esl_plan* p, int64_t info[8]);
 esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 						  const esl_solve_opts* opts, esl_solve_report* report);
 
+/* Autotuner: runs this batch under each candidate schedule -- per-set solve with
+ * the system in LDS, per-set from global memory, device-wide -- and restart length
+ * 16 / 32 / 64, keeps the fastest that meets opts->tol in the plan, and returns it
+ * (choice[0] = schedule 0/1/2, choice[1] = restart; *best_ms = its time per batch).
+ * d_final receives the batch's results.  Later esl_plan_solve / esl_plan_solve_many
+ * calls with opts->restart == 0 use the stored choice (esl_plan_query info[7] =
+ * 1000 * schedule + restart, or -1 before tuning).  No reference counterpart. */
+esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms);
+
 /* Several plans (distinct networks, e.g. of uneven size) in one call: plan i solves
  * n_sets[i] rate sets d_rates[i] -> d_final[i]; the per-set solves of all plans run
  * concurrently on forked streams (opts->graph = 1: captured once as one HIP graph and

@@ -371,7 +371,8 @@ class solve_plan:
     def info(self):
         a = np.zeros(8, dtype=np.int64)
         check(lib().esl_plan_query(self.handle, a.ctypes.data_as(C.POINTER(C.c_int64))))
-        keys = ["unknowns", "terminal_unknowns", "ambiguous", "terminals_reached", "terminal_segments", "nnz", "n_t"]
+        keys = ["unknowns", "terminal_unknowns", "ambiguous", "terminals_reached", "terminal_segments", "nnz", "n_t",
+                "tuned"]
         return dict(zip(keys, a.tolist()))
 
     def solve(self, n_sets, d_rates, d_final, **opts):
@@ -384,6 +385,17 @@ class solve_plan:
         check(lib().esl_plan_solve(self.context.handle, self.handle, n_sets, d_rates, d_final, C.byref(o),
                                    C.byref(rep)), self.context.handle)
         return rep.as_dict()
+
+    def autotune(self, n_sets, d_rates, d_final, **opts):
+        """Picks the fastest schedule for this batch (include/esl.h esl_plan_autotune);
+        later solves with restart=0 use it.  Returns (schedule, restart, ms)."""
+        o = solve_opts(**{k: v for k, v in opts.items() if k != "graph"})
+        o.graph = int(opts.get("graph", 0))
+        ch = (C.c_int32 * 2)()
+        ms = C.c_double()
+        check(lib().esl_plan_autotune(self.context.handle, self.handle, n_sets, d_rates, d_final, C.byref(o), ch,
+                                      C.byref(ms)), self.context.handle)
+        return int(ch[0]), int(ch[1]), float(ms.value)
 
     @staticmethod
     def solve_many(context, plans, n_sets, d_rates, d_finals, **opts):

@@ -598,6 +598,24 @@ esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d
 	});
 }
 
+esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms)
+{
+	return guarded(ctx, [&] {
+		require(ctx && p && d_rates && d_final && choice && best_ms, ESL_E_ARG, "null argument");
+		require(p->ctx == ctx, ESL_E_ARG, "plan belongs to another context");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		set_device(ctx);
+		esl_solve_opts o;
+		esl_solve_opts_default(&o);
+		if (opts)
+			o = *opts;
+		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		require(!o.full, ESL_E_ARG, "opts.full is not available on plans; use esl_solve");
+		esl::plan_autotune(ctx->c, p->p, n_sets, d_rates, d_final, o, choice, best_ms);
+	});
+}
+
 esl_status esl_plan_solve_many(esl_ctx* ctx, int n_plans, esl_plan* const* plans, const int* n_sets,
 							   const double* const* d_rates, double* const* d_final, const esl_solve_opts* opts,
 							   esl_solve_report* reports)

@@ -44,6 +44,7 @@
 // while the sets stream through.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <memory>
 
@@ -2399,7 +2400,7 @@ struct per_set_scratch_t
 
 // one launch: GMRES of every set, one workgroup per set (no host synchronisation)
 void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs, double* z,
-					const esl_solve_opts& o, per_set_scratch_t& sc)
+					const esl_solve_opts& o, per_set_scratch_t& sc, bool allow_lds = true)
 {
 	const int B = S.B;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
@@ -2408,7 +2409,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const nbr_t G = nb.view();
 	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get(), sc.stl.get() };
 	const double by = gmres_set_bytes(S, nb.nnz);
-	if (ilu && S.nU <= lds_max_unknowns && nb.nnz < 65536)
+	if (allow_lds && ilu && S.nU <= lds_max_unknowns && nb.nnz < 65536)
 	{
 		// LDS-resident kernel when its structure, pivots and vector fit
 		const int psize = o.precision == 1 ? 4 : 8;
@@ -2523,6 +2524,14 @@ struct plan_t
 	per_set_scratch_t scr;
 	int anchor_sweeps = 32;
 	int64_t epoch = 0; // bumped whenever a per-run buffer is reallocated
+	// schedule chosen by plan_autotune, used when a solve asks for restart 0
+	struct tune_t
+	{
+		bool set = false;
+		int path = 0; // 0 per-set (LDS-resident when it fits), 1 per-set from global memory, 2 device-wide
+		int m = 32;
+	} tune;
+	bool allow_lds = true; // the schedule of the run being enqueued
 	double ms[3] = { 0, 0, 0 };
 	// captured numeric sequence and the arguments it was captured with
 	struct gkey_t
@@ -2535,11 +2544,12 @@ struct plan_t
 		double tol = 0, inner_rtol = 0;
 		int max_iters = 0;
 		int64_t epoch = 0;
+		bool allow_lds = true;
 		bool operator==(const gkey_t& o) const
 		{
 			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
 				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && inner_rtol == o.inner_rtol &&
-				   max_iters == o.max_iters && epoch == o.epoch;
+				   max_iters == o.max_iters && epoch == o.epoch && allow_lds == o.allow_lds;
 		}
 	};
 	gkey_t gkey{};
@@ -2566,7 +2576,7 @@ struct plan_t
 
 
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
-							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr)
+							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr, bool device_wide = false)
 {
 	krylov_result_t kr;
 	const int64_t nU = S.nU;
@@ -2574,7 +2584,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	const double tol = opts.tol > 0 ? opts.tol : 1e-13;
 	const int max_iters = opts.max_iters > 0 ? opts.max_iters : 20000;
 	const int restart = std::min(std::max(opts.restart > 0 ? opts.restart : 32, 2), 64);
-	if (nU <= per_set_max_unknowns(B))
+	if (!device_wide && nU <= per_set_max_unknowns(B))
 	{
 		const int m = std::min(restart, pm_max);
 		nbr_store_t local;
@@ -2884,7 +2894,7 @@ void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_f
 {
 	plan_enqueue_setup(c, P, B, d_rates, o);
 	if (P.nU)
-		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr);
+		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr, P.allow_lds);
 	plan_enqueue_post(c, P, B, d_rates, d_final);
 }
 
@@ -2931,8 +2941,12 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	if (o.max_iters <= 0)
 		o.max_iters = 20000;
 	timer_t tm_(c);
+	const bool tuned = o.restart == 0 && P.tune.set;
+	if (tuned)
+		o.restart = P.tune.m;
+	P.allow_lds = !(tuned && P.tune.path == 1);
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
-	const bool per_set = P.nU == 0 || P.nU <= per_set_max_unknowns(B);
+	const bool per_set = P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B));
 	rep.terminals_reached = P.Tr;
 	rep.unknowns = P.nU;
 	rep.reachable = -1;
@@ -2943,8 +2957,8 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		const bool use_graph = o.graph != 0;
 		if (use_graph)
 		{
-			const plan_t::gkey_t key{ B,   d_rates,		 d_final,	  o.warm_pi, o.precision, o.preconditioner,
-									  m,   o.tol,		 o.inner_rtol, o.max_iters, P.epoch };
+			const plan_t::gkey_t key{ B, d_rates, d_final,	   o.warm_pi, o.precision, o.preconditioner, m,
+									  o.tol, o.inner_rtol, o.max_iters, P.epoch,   P.allow_lds };
 			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
 			{
 				P.drop_graph();
@@ -3000,7 +3014,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		// one large system: device-wide GMRES (host-driven restarts)
 		plan_reserve(P, B, 2, 0);
 		plan_enqueue_setup(c, P, B, d_rates, o);
-		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o, &P.nbr);
+		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o, &P.nbr, true);
 		plan_enqueue_post(c, P, B, d_rates, d_final);
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	}
@@ -3049,7 +3063,8 @@ struct multi_graph_t
 	std::vector<const double*> rates;
 	std::vector<double*> fin;
 	std::vector<int64_t> epochs;
-	int precision = 0, preconditioner = 0, m = 0, max_iters = 0;
+	std::vector<int> ms, lds;
+	int precision = 0, preconditioner = 0, max_iters = 0;
 	double tol = 0, inner_rtol = 0;
 	bool prof = false;
 	hipGraphExec_t exec = nullptr;
@@ -3058,8 +3073,8 @@ struct multi_graph_t
 	bool same(const multi_graph_t& o) const
 	{
 		return plans == o.plans && B == o.B && rates == o.rates && fin == o.fin && epochs == o.epochs &&
-			   precision == o.precision &&
-			   preconditioner == o.preconditioner && m == o.m && max_iters == o.max_iters && tol == o.tol &&
+			   ms == o.ms && lds == o.lds && precision == o.precision && preconditioner == o.preconditioner &&
+			   max_iters == o.max_iters && tol == o.tol &&
 			   inner_rtol == o.inner_rtol && prof == o.prof;
 	}
 	~multi_graph_t()
@@ -3097,21 +3112,24 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		o.tol = 1e-13;
 	if (o.max_iters <= 0)
 		o.max_iters = 20000;
-	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
 	std::vector<int> conc;
+	std::vector<esl_solve_opts> po(np, o); // per plan: the tuned restart length when asked for
 	for (int i = 0; i < np; ++i)
 	{
 		plan_t& P = *static_cast<plan_t*>(plans[i]);
 		reps[i] = esl_solve_report{};
-		if (P.nU == 0 || P.nU <= per_set_max_unknowns(B[i]))
+		const bool tuned = o_in.restart == 0 && P.tune.set;
+		P.allow_lds = !(tuned && P.tune.path == 1);
+		po[i].restart = std::min(std::max(tuned ? P.tune.m : (o.restart > 0 ? o.restart : 32), 2), pm_max);
+		if (P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B[i])))
 			conc.push_back(i);
 		else // one large system: its device-wide solver is driven from the host
-			plan_run(c, P, B[i], d_rates[i], d_final[i], o, reps[i]);
+			plan_run(c, P, B[i], d_rates[i], d_final[i], o_in, reps[i]);
 	}
 	if (conc.empty())
 		return;
 	for (int i : conc)
-		plan_reserve(*static_cast<plan_t*>(plans[i]), B[i], m, o.precision);
+		plan_reserve(*static_cast<plan_t*>(plans[i]), B[i], po[i].restart, o.precision);
 	ensure_aux(c, (int)conc.size());
 	timer_t tm_(c);
 	auto enqueue_all = [&] {
@@ -3124,7 +3142,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 			c.stream = c.aux[k];
 			try
 			{
-				plan_enqueue(c, *static_cast<plan_t*>(plans[i]), B[i], d_rates[i], d_final[i], o);
+				plan_enqueue(c, *static_cast<plan_t*>(plans[i]), B[i], d_rates[i], d_final[i], po[i]);
 			}
 			catch (...)
 			{
@@ -3146,10 +3164,11 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 			key.rates.push_back(d_rates[i]);
 			key.fin.push_back(d_final[i]);
 			key.epochs.push_back(static_cast<plan_t*>(plans[i])->epoch);
+			key.ms.push_back(po[i].restart);
+			key.lds.push_back(static_cast<plan_t*>(plans[i])->allow_lds ? 1 : 0);
 		}
 		key.precision = o.precision;
 		key.preconditioner = o.preconditioner;
-		key.m = m;
 		key.max_iters = o.max_iters;
 		key.tol = o.tol;
 		key.inner_rtol = o.inner_rtol;
@@ -3214,8 +3233,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		rep.reachable = -1;
 		if (P.nU > 0)
 		{
-			esl_solve_opts of = o;
-			of.restart = m;
+			const esl_solve_opts& of = po[i];
 			const krylov_result_t kr =
 				finish_per_set(c, P.sys(B[i], d_rates[i]), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), of, P.scr);
 			if (kr.fp64_fallback)
@@ -3235,6 +3253,78 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 	}
 }
 
+// Times candidate schedules of a plan on the caller's own batch and keeps the
+// fastest one that meets the tolerance: per-set LDS-resident / per-set from
+// global memory / device-wide, restart length 16 / 32 / 64.  Each candidate
+// runs once untimed (graph capture, buffer growth) and twice timed (best of
+// two).  Later solves use the choice when they ask for restart 0.
+void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
+				   int32_t choice[2], double* best_ms)
+{
+	plan_t& P = *static_cast<plan_t*>(p);
+	esl_solve_opts o = o_in;
+	o.restart = 0;
+	struct cand_t
+	{
+		int path, m;
+	};
+	std::vector<cand_t> cands;
+	const bool ilu = o.preconditioner != 1;
+	if (P.nU == 0)
+		cands.push_back({ 0, 32 });
+	else
+		for (int m : { 16, 32, 64 })
+		{
+			cands.push_back({ 0, m });
+			if (ilu && P.nU <= lds_max_unknowns) // otherwise path 1 is path 0
+				cands.push_back({ 1, m });
+			cands.push_back({ 2, m });
+		}
+	const plan_t::tune_t saved = P.tune;
+	plan_t::tune_t best_t{};
+	double best = 1e300;
+	for (const cand_t& cd : cands)
+	{
+		P.tune = plan_t::tune_t{ true, cd.path, cd.m };
+		try
+		{
+			esl_solve_report rep{};
+			plan_run(c, P, B, d_rates, d_final, o, rep);
+			double t = 1e300;
+			for (int r = 0; r < 2; ++r)
+			{
+				const auto t0 = std::chrono::steady_clock::now();
+				plan_run(c, P, B, d_rates, d_final, o, rep);
+				t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+			}
+			if (t < best)
+			{
+				best = t;
+				best_t = P.tune;
+			}
+		}
+		catch (const error& e)
+		{
+			// a schedule that does not converge or does not fit is skipped
+			if (e.code != ESL_E_NUMERIC && e.code != ESL_E_OOM)
+			{
+				P.tune = saved;
+				throw;
+			}
+			(void)hipGetLastError();
+		}
+	}
+	if (!best_t.set)
+	{
+		P.tune = saved;
+		throw error(ESL_E_NUMERIC, "autotune: no schedule reached the tolerance");
+	}
+	P.tune = best_t;
+	choice[0] = P.tune.path;
+	choice[1] = P.tune.m;
+	*best_ms = best;
+}
+
 void plan_info(const void* p, int64_t info[8])
 {
 	const plan_t& P = *static_cast<const plan_t*>(p);
@@ -3245,7 +3335,7 @@ void plan_info(const void* p, int64_t info[8])
 	info[4] = P.n_seg;
 	info[5] = P.nbr.nnz;
 	info[6] = P.g ? P.g->n_t : 0;
-	info[7] = 0;
+	info[7] = P.tune.set ? (int64_t)P.tune.path * 1000 + P.tune.m : -1;
 }
 
 // solution_term and solution_nonterm in the reference layout (one rate set).

@@ -201,3 +201,23 @@ def test_solve_many_matches_single_plans(ctx, graph):
             ref = singles[k].download()
             assert np.abs(got - ref).sum(axis=1).max() <= 1e-10, (rep_i, k)
             assert reps[k]["residual"] <= 1e-12
+
+
+@pytest.mark.parametrize("cfg,seed,B", [("c0", 1, 8), ("c2", 11, 4)])
+def test_autotune_choice_keeps_results(ctx, cfg, seed, B):
+    net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    d_ref = DeviceArray(ctx, (B, n), np.float64)
+    d_out = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    plan.solve(B, dr.ptr, d_ref.ptr, precision=0, tol=1e-13)
+    assert plan.info()["tuned"] == -1
+    path, restart, ms = plan.autotune(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12, graph=1)
+    assert path in (0, 1, 2) and restart in (16, 32, 64) and ms > 0
+    assert plan.info()["tuned"] == 1000 * path + restart
+    for graph in (0, 1):
+        rep = plan.solve(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12, restart=0, graph=graph)
+        assert rep["residual"] <= 1e-12
+        assert np.abs(d_out.download() - d_ref.download()).sum(axis=1).max() <= 1e-10
