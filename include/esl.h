@@ -156,6 +156,7 @@ typedef struct
 	int fp64_fallback;	 /* precision 1: rate sets whose FP32 cycles stalled and that were finished in FP64 */
 	double ms_total;	 /* device time of the call */
 	double ms_stage[8];	 /* reach, terminal masks, setup, terminal solve, transient solve, final, -, - */
+	int64_t iterations_sum; /* Krylov iterations summed over the rate sets */
 } esl_solve_report;
 
 void esl_solve_opts_default(esl_solve_opts* o);
@@ -210,6 +211,12 @@ esl_status esl_plan_query(const esl_plan* p, int64_t info[8]);
  * call and replayed while n_sets, the pointers and the options stay the same. */
 esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 						  const esl_solve_opts* opts, esl_solve_report* report);
+
+/* d_out[n_sets][n_t] = d_final[set][reordered_vertices[i]] for i < n_t: the final
+ * vectors on the terminal states in canonical order -- the layout of opts->warm_pi
+ * and of the surrogate's targets (include/esl_mlp.h).  No reference counterpart. */
+esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets, const double* d_final,
+									double* d_out);
 
 /* Autotuner: runs this batch under each candidate schedule -- per-set solve with
  * the system in LDS, per-set from global memory, device-wide -- and restart length

@@ -33,6 +33,9 @@ esl_status esl_mlp_get_params(esl_ctx* ctx, const esl_mlp* m, float* host_out);
 esl_status esl_mlp_set_params(esl_ctx* ctx, esl_mlp* m, const float* host_in);
 /* probabilities [rows][n_out] of inputs [rows][n_in] */
 esl_status esl_mlp_forward(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x, float* d_probs);
+/* the prediction as a warm start: probabilities of inputs [rows][n_in], widened to
+ * FP64 [rows][n_out] -- the layout of esl_solve_opts.warm_pi when n_out = n_t */
+esl_status esl_mlp_predict_warm(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x, double* d_warm);
 /* gradients of the mean cross-entropy against targets [rows][n_out] (rows sum to 1),
  * copied to host_grads (num_params floats) when it is not NULL */
 esl_status esl_mlp_gradients(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x, const float* d_target,

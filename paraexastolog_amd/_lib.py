@@ -48,6 +48,7 @@ class SolveReport(C.Structure):
         ("fp64_fallback", C.c_int),
         ("ms_total", C.c_double),
         ("ms_stage", C.c_double * 8),
+        ("iterations_sum", C.c_int64),
     ]
 
     def as_dict(self):
@@ -59,6 +60,7 @@ class SolveReport(C.Structure):
             "terminals_reached": self.terminals_reached,
             "sets_iterated": self.sets_iterated,
             "fp64_fallback": self.fp64_fallback,
+            "iterations_sum": self.iterations_sum,
             "ms_total": self.ms_total,
             "ms_stage": list(self.ms_stage)[:5],
         }
@@ -105,6 +107,7 @@ SIGNATURES = [
     ("esl_plan_destroy", None, [_VP]),
     ("esl_plan_query", C.c_int, [_VP, _P(C.c_int64)]),
     ("esl_plan_solve", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
+    ("esl_plan_gather_terminal", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_plan_autotune", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(C.c_int32), _P(C.c_double)]),
     ("esl_plan_solve_many", C.c_int, [_VP, C.c_int, _P(_VP), _P(C.c_int), _P(_VP), _P(_VP), _P(SolveOpts),
                                       _P(SolveReport)]),
@@ -114,6 +117,7 @@ SIGNATURES = [
     ("esl_mlp_get_params", C.c_int, [_VP, _VP, _P(C.c_float)]),
     ("esl_mlp_set_params", C.c_int, [_VP, _VP, _P(C.c_float)]),
     ("esl_mlp_forward", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
+    ("esl_mlp_predict_warm", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_mlp_gradients", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(C.c_float), _P(C.c_float)]),
     ("esl_mlp_train_step", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.c_float, _P(C.c_float)]),
     ("esl_profile_stat", C.c_int, [_VP, C.c_int, C.c_char_p, C.c_int, _P(C.c_int64), _P(C.c_double),

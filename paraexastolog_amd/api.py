@@ -386,6 +386,11 @@ class solve_plan:
                                    C.byref(rep)), self.context.handle)
         return rep.as_dict()
 
+    def gather_terminal(self, n_sets, d_final, d_out):
+        """d_out[n_sets][n_t] = final vectors on the terminal states (warm-start layout)."""
+        check(lib().esl_plan_gather_terminal(self.context.handle, self.handle, n_sets, d_final, d_out),
+              self.context.handle)
+
     def autotune(self, n_sets, d_rates, d_final, **opts):
         """Picks the fastest schedule for this batch (include/esl.h esl_plan_autotune);
         later solves with restart=0 use it.  Returns (schedule, restart, ms)."""

@@ -598,6 +598,18 @@ esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d
 	});
 }
 
+esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets, const double* d_final,
+									double* d_out)
+{
+	return guarded(ctx, [&] {
+		require(ctx && p && d_final && d_out, ESL_E_ARG, "null argument");
+		require(p->ctx == ctx, ESL_E_ARG, "plan belongs to another context");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		set_device(ctx);
+		esl::plan_gather_terminal(ctx->c, p->p, n_sets, d_final, d_out);
+	});
+}
+
 esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms)
 {

@@ -111,6 +111,13 @@ __global__ __launch_bounds__(gemm_block) void k_gemm_bf16(int M, int N, int K, c
 }
 
 // per row: P = softmax(Z); loss += -sum t log p; dZ = (P - T) / rows
+__global__ __launch_bounds__(256) void k_widen(int64_t count, const float* __restrict__ a, double* __restrict__ b)
+{
+	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < count)
+		b[i] = (double)a[i];
+}
+
 __global__ __launch_bounds__(256) void k_softmax_xent(int rows, int n, const float* __restrict__ Z,
 													  const float* __restrict__ Tg, float* __restrict__ P,
 													  float* __restrict__ dZ, float* __restrict__ loss_rows)
@@ -375,6 +382,26 @@ esl_status esl_mlp_forward(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x,
 		const int L = (int)m->dims.size() - 1;
 		esl::k_softmax_xent<<<rows, 256, 0, c.stream>>>(rows, m->n_out, m->pre.get() + m->act_off[L], nullptr,
 														 d_probs, nullptr, nullptr);
+		ESL_HIP(hipGetLastError());
+		ESL_HIP(hipStreamSynchronize(c.stream));
+	});
+}
+
+esl_status esl_mlp_predict_warm(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x, double* d_warm)
+{
+	return mlp_guard(ctx, [&] {
+		if (!ctx || !m || !d_x || !d_warm || rows < 1)
+			throw esl::error(ESL_E_ARG, "null argument");
+		esl::ctx_t& c = esl::ctx_impl(ctx);
+		ensure_rows(m, rows);
+		ESL_HIP(hipMemcpyAsync(m->act.get(), d_x, (size_t)rows * m->n_in * sizeof(float), hipMemcpyDeviceToDevice,
+							   c.stream));
+		forward(c, m, rows);
+		const int L = (int)m->dims.size() - 1;
+		esl::k_softmax_xent<<<rows, 256, 0, c.stream>>>(rows, m->n_out, m->pre.get() + m->act_off[L], nullptr,
+														 m->probs.get(), nullptr, nullptr);
+		const int64_t cnt = (int64_t)rows * m->n_out;
+		esl::k_widen<<<esl::ceil_div(cnt, 256), 256, 0, c.stream>>>(cnt, m->probs.get(), d_warm);
 		ESL_HIP(hipGetLastError());
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	});

@@ -2348,6 +2348,7 @@ struct krylov_result_t
 	int sets_iterated = 0; // sets whose initial iterate missed the tolerance
 	int fp64_fallback = 0; // sets finished in FP64 after their FP32 cycles stalled
 	int missed = 0;		   // sets above the tolerance
+	int64_t iters_sum = 0; // iterations summed over the sets
 };
 
 // scratch of the per-set GMRES kernel, grown on demand and reused
@@ -2389,6 +2390,7 @@ struct per_set_scratch_t
 			kr.cycles = std::max(kr.cycles, h_cyc[b]);
 			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
 			kr.missed += (h_res[b] <= tol) ? 0 : 1;
+			kr.iters_sum += h_its[b];
 		}
 		if (check && !(kr.resid <= tol))
 			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
@@ -2501,6 +2503,7 @@ krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, 
 	kr.resid = k2.resid;
 	kr.iters += k2.iters;
 	kr.cycles += k2.cycles;
+	kr.iters_sum += k2.iters_sum;
 	return kr;
 }
 
@@ -2631,6 +2634,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 			kr.cycles += gm.cycles;
 		}
 		kr.sets_iterated = kr.cycles > 0 ? B : 0;
+		kr.iters_sum = (int64_t)kr.iters * B; // the sets iterate together
 		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 1);
 		ESL_HIP(hipGetLastError());
 	}
@@ -3023,6 +3027,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	rep.refinements = kr.cycles;
 	rep.sets_iterated = kr.sets_iterated;
 	rep.fp64_fallback = kr.fp64_fallback;
+	rep.iterations_sum = kr.iters_sum;
 	rep.ms_stage[3] = tm_.lap();
 	rep.ms_total = rep.ms_stage[3];
 }
@@ -3243,6 +3248,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 			rep.refinements = kr.cycles;
 			rep.sets_iterated = kr.sets_iterated;
 			rep.fp64_fallback = kr.fp64_fallback;
+			rep.iterations_sum = kr.iters_sum;
 		}
 	}
 	const double ms = tm_.lap();
@@ -3251,6 +3257,29 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		reps[i].ms_stage[3] = ms;
 		reps[i].ms_total = ms;
 	}
+}
+
+// out[b][i] = final[b][reordered[i]], i < n_t: the final vectors on the terminal
+// states in canonical order (the layout of a warm start and of surrogate targets)
+__global__ __launch_bounds__(sblock) void k_gather_terminal(int64_t n, int64_t n_t, const int32_t* __restrict__ reordered,
+															const double* __restrict__ fin, double* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int64_t b = blockIdx.y;
+	if (i < n_t)
+		out[b * n_t + i] = fin[b * n + reordered[i]];
+}
+
+void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final, double* d_out)
+{
+	const plan_t& P = *static_cast<const plan_t*>(p);
+	const graph_t& g = *P.g;
+	if (g.n_t == 0)
+		return;
+	k_gather_terminal<<<dim3(ceil_div(g.n_t, sblock), B), sblock, 0, c.stream>>>(g.n, g.n_t, g.reordered.get(),
+																				  d_final, d_out);
+	ESL_HIP(hipGetLastError());
+	ESL_HIP(hipStreamSynchronize(c.stream));
 }
 
 // Times candidate schedules of a plan on the caller's own batch and keeps the

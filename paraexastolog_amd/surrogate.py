@@ -85,6 +85,10 @@ class Surrogate:
         self.forward_device(x.shape[0], dx.ptr, dp.ptr)
         return dp.download()
 
+    def predict_warm_device(self, rows, d_x, d_warm):
+        """FP64 warm start [rows][n_out] on the device (esl_mlp_predict_warm)."""
+        check(lib().esl_mlp_predict_warm(self.ctx.handle, self.handle, rows, d_x, d_warm), self.ctx.handle)
+
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
@@ -108,3 +112,53 @@ class Surrogate:
                 self.handle = None
         except Exception:
             pass
+
+
+def log_rates(rates):
+    """Network inputs: the 2N natural-log rates of each set, FP32."""
+    return np.log(np.asarray(rates, dtype=np.float64)).astype(np.float32)
+
+
+def terminal_targets(ctx, plan, rates, n, n_t, batch=1024, **opts):
+    """Solves every rate set with `plan` and returns the final vectors on the
+    terminal states in canonical order, [sets][n_t] FP64 -- the training targets
+    (they sum to 1 per set: all mass ends on terminal states)."""
+    rates = np.ascontiguousarray(rates, dtype=np.float64)
+    S = rates.shape[0]
+    batch = min(batch, S)
+    out = np.empty((S, n_t))
+    dr = DeviceArray(ctx, (batch, rates.shape[1]), np.float64)
+    df = DeviceArray(ctx, (batch, n), np.float64)
+    dt = DeviceArray(ctx, (batch, n_t), np.float64)
+    for a in range(0, S, batch):
+        k = min(S, a + batch) - a
+        dr.upload(rates[a:a + k])
+        plan.solve(k, dr.ptr, df.ptr, **opts)
+        plan.gather_terminal(k, df.ptr, dt.ptr)
+        out[a:a + k] = dt.download()[:k]
+    return out
+
+
+def train(sur, X, T, epochs, batch=512, lr=2e-3, lr_final=1e-4):
+    """Adam on the mean cross-entropy over minibatches of consecutive rows (the
+    rate sets are i.i.d., so no shuffling is needed); the learning rate decays
+    geometrically from lr to lr_final.  Returns the mean loss of each epoch."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    rows, n_in = X.shape
+    n_out = T.shape[1]
+    batch = min(batch, rows)
+    steps = rows // batch
+    dX = DeviceArray.from_host(sur.ctx, X)
+    dT = DeviceArray.from_host(sur.ctx, T)
+    total = max(epochs * steps - 1, 1)
+    hist = []
+    k = 0
+    for _ in range(epochs):
+        acc = 0.0
+        for s in range(steps):
+            cur = lr * (lr_final / lr) ** (k / total)
+            acc += sur.train_step_device(batch, dX.offset(s * batch * n_in), dT.offset(s * batch * n_out), cur)
+            k += 1
+        hist.append(acc / steps)
+    return hist
