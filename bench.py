@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--graph", type=int, default=1, help="replay the numeric sequence as a HIP graph")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="residual reduction an FP32 cycle aims for (library default when omitted)")
+    ap.add_argument("--autotune", type=int, default=1, help="pick the schedule with esl_plan_autotune first")
+    ap.add_argument("--workload", default="n20", choices=["n20", "n16", "uneven", "n22", "warm"],
+                    help="n20 (default) is the headline metric; the others print their own JSON line")
+    ap.add_argument("--epochs", type=int, default=30, help="training epochs of the warm workload")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the run, copy 1 GiB device to device three times (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -97,6 +101,10 @@ def main():
 
     from paraexastolog_amd import cu_context, model_builder, solve_opts, synth, transition_table
     from paraexastolog_amd._lib import SolveReport, check, lib
+
+    if a.workload != "n20":
+        WORKLOADS[a.workload](a, cu_context(0))
+        return
 
     L = lib()
     net, rng = synth.config("c1", a.seed)
@@ -151,8 +159,8 @@ def main():
     info = np.zeros(8, dtype=np.int64)
     check(L.esl_plan_query(pl, info.ctypes.data_as(C.POINTER(C.c_int64))))
 
-    def solve_batch(count, d_r, d_out, precision, tol, graph=0):
-        o = solve_opts(precision=precision, tol=tol, inner_rtol=a.inner_rtol)
+    def solve_batch(count, d_r, d_out, precision, tol, graph=0, restart=32):
+        o = solve_opts(precision=precision, tol=tol, inner_rtol=a.inner_rtol, restart=restart)
         o.graph = graph
         rep = SolveReport()
         check(L.esl_plan_solve(ctx.handle, pl, count, d_r, d_out, C.byref(o), C.byref(rep)), ctx.handle)
@@ -190,12 +198,25 @@ def main():
     solve_batch(1, d_rates, d_plain, 0, 1e-13)
     plain_full_s = float(np.median(full_times))
 
-    # ---- batched + refined: warmup, then K timed steps
+    # ---- batched + refined: the autotuner picks the schedule on this batch, then
+    # warmup and K timed steps with the tuned schedule (restart 0 = tuned)
     tol = 1e-12
+    schedule = None
+    restart = 32
+    if a.autotune:
+        o = solve_opts(precision=a.precision, tol=tol, inner_rtol=a.inner_rtol)
+        o.graph = a.graph
+        ch = (C.c_int32 * 2)()
+        best = C.c_double()
+        t = time.perf_counter()
+        check(L.esl_plan_autotune(ctx.handle, pl, B, d_rates, d_final, C.byref(o), ch, C.byref(best)), ctx.handle)
+        schedule = {"path": ["per-set LDS-resident", "per-set global", "device-wide"][ch[0]], "restart": int(ch[1]),
+                    "best_ms": best.value, "tuning_s": time.perf_counter() - t}
+        restart = 0
     # kernel timing events on (they are captured into the HIP graph as event nodes)
     check(L.esl_profile(ctx.handle, 1), ctx.handle)
     for _ in range(max(a.warmup, 1)):  # the first call captures the HIP graph
-        solve_batch(B, d_rates, d_final, a.precision, tol, graph=a.graph)
+        solve_batch(B, d_rates, d_final, a.precision, tol, graph=a.graph, restart=restart)
     try:
         import torch
 
@@ -207,7 +228,7 @@ def main():
     reports = []
     t = time.perf_counter()
     for _ in range(a.steps):
-        reports.append(solve_batch(B, d_rates, d_final, a.precision, tol, graph=a.graph).as_dict())
+        reports.append(solve_batch(B, d_rates, d_final, a.precision, tol, graph=a.graph, restart=restart).as_dict())
     sync()
     elapsed = time.perf_counter() - t
     stats = []
@@ -293,7 +314,10 @@ def main():
         "accuracy": {"max_l1_vs_plain_fp64": max(l1), "max_rel_vs_plain_fp64": max(maxrel),
                      "plain_fp64_l1_vs_oracle": oracle_l1, "max_abs_sum_minus_1": float(sums),
                      "max_residual": max(r["residual"] for r in reports),
-                     "gmres_iterations": rep0["iterations"], "refinement_cycles": rep0["refinements"]},
+                     "gmres_iterations": rep0["iterations"], "refinement_cycles": rep0["refinements"],
+                     "mean_iterations_per_set": rep0["iterations_sum"] / B,
+                     "fp64_fallback_sets": rep0["fp64_fallback"]},
+        "schedule": schedule,
         "kernels": [{"name": s[0], "launches": s[1], "ms": s[2], "algorithmic_GB": s[3] / 1e9} for s in stats],
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "avg_launch_ms": avg_ms,
@@ -311,6 +335,235 @@ def main():
     if max(l1) > 1e-10:
         print(f"bench.py: accuracy bar missed: L1 {max(l1):.3e} > 1e-10", file=sys.stderr)
         sys.exit(1)
+
+
+# ---------------------------------------------------------------------------
+# further workloads (each prints its own JSON line; the default run is the one above)
+
+
+class _Topo:
+    """One network on the device: model, table + graph, initial state, plan."""
+
+    def __init__(self, ctx, net, rates0):
+        from paraexastolog_amd import initial_state, model_builder, solve_plan, transition_graph, transition_table
+        from paraexastolog_amd.surrogate import DeviceArray
+
+        self.net = net
+        self.n = 1 << net.N
+        t0 = time.perf_counter()
+        self.model = model_builder().parse_model(net.bnet())
+        self.table = transition_table(ctx, self.model, rates0)
+        self.table.construct_table()
+        self.graph = transition_graph(ctx, self.table)
+        self.graph.reorganize_vertices()
+        fixed, vals = net.fixed_spec()
+        self.x0 = initial_state(self.model.nodes, fixed, vals).state
+        self.d_x0 = DeviceArray.from_host(ctx, self.x0)
+        self.plan = solve_plan(ctx, self.table, self.d_x0.ptr)
+        self.setup_s = time.perf_counter() - t0
+        self.n_t = int(self.graph.sccs_offsets[self.graph.terminals_count])
+        self.info = self.plan.info()
+
+
+def _timed(fn, steps):
+    fn()  # warm-up (graph capture, buffer growth)
+    t = time.perf_counter()
+    rep = None
+    for _ in range(steps):
+        rep = fn()
+    return (time.perf_counter() - t) / steps, rep
+
+
+def _errors(x, ref):
+    """max L1 and max relative error (over nonzero reference entries) per row."""
+    l1 = np.abs(x - ref).sum(axis=1)
+    nz = ref != 0
+    rel = np.where(nz, np.abs(x - ref) / np.where(nz, np.abs(ref), 1.0), 0.0).max(axis=1)
+    return float(l1.max()), float(np.median(l1)), float(rel.max())
+
+
+def workload_n16(a, ctx):
+    """One N=16 topology (K=2, generator v1 seed 11: one terminal SCC of 2048
+    states), 1024 rate sets sharing the plan: FP64 and FP32+refinement, HIP graph
+    off/on, autotuned schedule; against the plain per-solve FP64 path."""
+    from paraexastolog_amd import synth
+    from paraexastolog_amd.surrogate import DeviceArray
+
+    seed = 11 if a.seed == 0 else a.seed
+    net, rng = synth.config("c2", seed)
+    B = a.sets
+    rates = synth.rate_sets(rng, net.N, B)
+    tp = _Topo(ctx, net, rates[0])
+    dr = DeviceArray.from_host(ctx, rates)
+    df = DeviceArray(ctx, (B, tp.n), np.float64)
+    dp = DeviceArray(ctx, (8, tp.n), np.float64)
+    plain = []
+    for k in range(8):
+        t = time.perf_counter()
+        tp.plan.solve(1, dr.offset(2 * net.N * k), dp.offset(tp.n * k), precision=0, tol=1e-13)
+        plain.append(time.perf_counter() - t)
+    plain_s = float(np.median(plain[1:]))
+    ref = dp.download()
+    out = {}
+    for prec in (0, 1):
+        for graph in (0, 1):
+            sec, rep = _timed(lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=prec, tol=1e-12, graph=graph),
+                              a.steps)
+            l1, l1_med, rel = _errors(df.download()[:8], ref)
+            out[f"{'fp32+fp64' if prec else 'fp64'}_graph{graph}"] = {
+                "sets_per_s": B / sec, "ms_per_batch": 1000 * sec, "max_l1_vs_plain": l1, "max_rel_vs_plain": rel,
+                "mean_iterations": rep["iterations_sum"] / B}
+    path, restart, best_ms = tp.plan.autotune(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1)
+    sec, rep = _timed(lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1, restart=0), a.steps)
+    out["autotuned"] = {"sets_per_s": B / sec, "ms_per_batch": 1000 * sec,
+                        "schedule": ["per-set LDS-resident", "per-set global", "device-wide"][path], "restart": restart}
+    best = max(v["sets_per_s"] for v in out.values())
+    print(json.dumps({
+        "metric": "rate sets/s, N=16 topology, batch of shared-plan rate sets", "value": best, "unit": "sets/s",
+        "workload": f"N=16 K=2 generator v1 seed {seed}; {B} log-uniform [0.1,10] rate sets; tol 1e-12",
+        "plan": tp.info, "setup_ms_table_graph_plan": 1000 * tp.setup_s,
+        "plain_fp64": {"sets_per_s": 1.0 / plain_s, "ms_per_solve": 1000 * plain_s},
+        "speedup_vs_plain_fp64": best * plain_s, "variants": out}))
+
+
+def workload_uneven(a, ctx):
+    """Eight networks of N = 12..18, 18 (K=2), 128 rate sets each, in one call
+    (esl_plan_solve_many, concurrent streams, one HIP graph) against the same
+    plans solved one after another."""
+    from paraexastolog_amd import solve_plan, synth
+    from paraexastolog_amd.surrogate import DeviceArray
+
+    per = 128
+    nets = synth.uneven(a.seed)
+    tops, drs, dfs, rates_all = [], [], [], []
+    for net, rng in nets:
+        rates = synth.rate_sets(rng, net.N, per)
+        tp = _Topo(ctx, net, rates[0])
+        tops.append(tp)
+        rates_all.append(rates)
+        drs.append(DeviceArray.from_host(ctx, rates))
+        dfs.append(DeviceArray(ctx, (per, tp.n), np.float64))
+    plans = [t.plan for t in tops]
+    res = {}
+    for prec in (0, 1):
+        seq_s, _ = _timed(lambda: [t.plan.solve(per, d.ptr, f.ptr, precision=prec, tol=1e-12, graph=1)
+                                   for t, d, f in zip(tops, drs, dfs)], a.steps)
+        seq = [f.download() for f in dfs]
+        for graph in (0, 1):
+            sec, reps = _timed(lambda: solve_plan.solve_many(ctx, plans, [per] * 8, [d.ptr for d in drs],
+                                                             [f.ptr for f in dfs], precision=prec, tol=1e-12,
+                                                             graph=graph), a.steps)
+            diff = max(np.abs(f.download() - s_).sum(axis=1).max() for f, s_ in zip(dfs, seq))
+            res[f"{'fp32+fp64' if prec else 'fp64'}_graph{graph}"] = {
+                "sets_per_s": 8 * per / sec, "ms": 1000 * sec, "sequential_sets_per_s": 8 * per / seq_s,
+                "max_l1_vs_sequential": float(diff), "max_residual": max(r["residual"] for r in reps)}
+    best = max(v["sets_per_s"] for v in res.values())
+    print(json.dumps({
+        "metric": "rate sets/s, 8 uneven networks x 128 sets in one call", "value": best, "unit": "sets/s",
+        "workload": f"N=12..18,18 K=2 generator v1 seeds 1000*{a.seed}+k; 128 rate sets each; tol 1e-12",
+        "networks": [{"N": t.net.N, **{k: t.info[k] for k in ("unknowns", "ambiguous", "terminals_reached")}}
+                     for t in tops], "variants": res}))
+
+
+def workload_n22(a, ctx):
+    """N=22, K=3 with three spliced negative-feedback 3-cycles (generator v1
+    seed 8: one terminal SCC of 221,184 states), 64 rate sets: FP32 Krylov +
+    FP64 refinement against direct FP64, errors of every result."""
+    from paraexastolog_amd import synth
+    from paraexastolog_amd.surrogate import DeviceArray
+
+    seed = 8 if a.seed == 0 else a.seed
+    net, rng = synth.config("c3", seed)
+    B = min(a.sets, 64)
+    rates = synth.rate_sets(rng, net.N, B)
+    tp = _Topo(ctx, net, rates[0])
+    dr = DeviceArray.from_host(ctx, rates)
+    d64 = DeviceArray(ctx, (B, tp.n), np.float64)
+    d32 = DeviceArray(ctx, (B, tp.n), np.float64)
+    # 64 sets of 2e5 unknowns: the device-wide schedule (every vector operation over all sets at once)
+    tp.plan.set_schedule(2, 32)
+    steps = max(1, a.steps // 3)
+    t64, r64 = _timed(lambda: tp.plan.solve(B, dr.ptr, d64.ptr, precision=0, tol=1e-12, restart=0), steps)
+    print(f"n22: fp64 {t64:.3f} s per batch", file=sys.stderr, flush=True)
+    t32, r32 = _timed(lambda: tp.plan.solve(B, dr.ptr, d32.ptr, precision=1, tol=1e-12, restart=0), steps)
+    x64 = d64.download()
+    x32 = d32.download()
+    l1 = np.abs(x32 - x64).sum(axis=1)
+    nz = x64 != 0
+    rel = np.where(nz, np.abs(x32 - x64) / np.where(nz, np.abs(x64), 1.0), 0.0).max(axis=1)
+    print(json.dumps({
+        "metric": "rate sets/s, N=22 mixed precision (FP32 Krylov + FP64 refinement vs FP64)",
+        "value": B / t32, "unit": "sets/s",
+        "workload": f"N=22 K=3 + 3 spliced negative-feedback 3-cycles, generator v1 seed {seed}, 5 fixed nodes; "
+                    f"{B} rate sets; tol 1e-12",
+        "plan": tp.info, "terminal_scc_states": tp.n_t,
+        "fp64": {"sets_per_s": B / t64, "s_per_batch": t64, "mean_iterations": r64["iterations_sum"] / B},
+        "fp32_refined": {"sets_per_s": B / t32, "s_per_batch": t32, "mean_iterations": r32["iterations_sum"] / B,
+                         "refinement_cycles": r32["refinements"], "fp64_fallback_sets": r32["fp64_fallback"]},
+        "speedup": t64 / t32,
+        "errors_fp32_vs_fp64": {"l1_max": float(l1.max()), "l1_median": float(np.median(l1)),
+                                "max_rel_max": float(rel.max()), "max_rel_median": float(np.median(rel))},
+        "sums": [float(np.abs(x64.sum(axis=1) - 1).max()), float(np.abs(x32.sum(axis=1) - 1).max())]}))
+
+
+def workload_warm(a, ctx):
+    """The learned warm start on the N=16 topology (seed 11, 2048 terminal
+    states): 50k pipeline-generated training solves, the MLP trained on the GPU,
+    4096 held-out rate sets solved cold and warm (prediction included in the
+    warm timing)."""
+    from paraexastolog_amd import synth
+    from paraexastolog_amd.surrogate import DeviceArray, Surrogate, log_rates, terminal_targets, train
+
+    seed = 11 if a.seed == 0 else a.seed
+    net, rng = synth.config("c2", seed)
+    n_train, n_test = 50000, 4096
+    rates = synth.rate_sets(rng, net.N, n_train + n_test)
+    tp = _Topo(ctx, net, rates[0])
+    t = time.perf_counter()
+    T = terminal_targets(ctx, tp.plan, rates, tp.n, tp.n_t, batch=2048, precision=1, tol=1e-12, graph=1)
+    gen_s = time.perf_counter() - t
+    X = log_rates(rates)
+    sur = Surrogate(ctx, 2 * net.N, tp.n_t, seed=3)
+    t = time.perf_counter()
+    hist = train(sur, X[:n_train], T[:n_train], epochs=a.epochs, batch=512, lr=1e-2)
+    train_s = time.perf_counter() - t
+    Tt = T[n_train:]
+    ent = float(-(np.where(Tt > 0, Tt * np.log(np.maximum(Tt, 1e-300)), 0.0)).sum(axis=1).mean())
+    dX = DeviceArray.from_host(ctx, X[n_train:])
+    dW = DeviceArray(ctx, (n_test, tp.n_t), np.float64)
+    dr = DeviceArray.from_host(ctx, rates[n_train:])
+    dc = DeviceArray(ctx, (n_test, tp.n), np.float64)
+    dw = DeviceArray(ctx, (n_test, tp.n), np.float64)
+    cold_s, cold = _timed(lambda: tp.plan.solve(n_test, dr.ptr, dc.ptr, precision=1, tol=1e-12, graph=1), a.steps)
+
+    def warm_step():
+        sur.predict_warm_device(n_test, dX.ptr, dW.ptr)
+        return tp.plan.solve(n_test, dr.ptr, dw.ptr, precision=1, tol=1e-12, graph=1, warm=dW.ptr)
+
+    warm_s, warm = _timed(warm_step, a.steps)
+    W = dW.download()
+    err = np.abs(W - Tt).sum(axis=1)
+    base = np.abs(T[:n_train].mean(axis=0) - Tt).sum(axis=1)
+    diff = float(np.abs(dc.download() - dw.download()).sum(axis=1).max())
+    n_par = sur.n_params
+    print(json.dumps({
+        "metric": "rate sets/s with vs without the learned warm start (N=16, held-out sets)",
+        "value": n_test / warm_s, "unit": "sets/s",
+        "workload": f"N=16 K=2 generator v1 seed {seed}; MLP 32-256-256-256-{tp.n_t} (GELU, softmax), BF16 MFMA / "
+                    f"FP32 accumulate, Adam; {n_train} training solves, {n_test} held out; tol 1e-12",
+        "data_generation": {"solves": n_train + n_test, "s": gen_s, "solves_per_s": (n_train + n_test) / gen_s},
+        "training": {"epochs": a.epochs, "steps": a.epochs * (n_train // 512), "s": train_s,
+                     "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "held_out_target_entropy": ent,
+                     "tflops": 6.0 * 512 * n_par * a.epochs * (n_train // 512) / train_s / 1e12},
+        "surrogate_error_l1": {"median": float(np.median(err)), "p90": float(np.percentile(err, 90)),
+                               "max": float(err.max()), "constant_predictor_median": float(np.median(base))},
+        "cold": {"sets_per_s": n_test / cold_s, "mean_iterations": cold["iterations_sum"] / n_test},
+        "warm": {"sets_per_s": n_test / warm_s, "mean_iterations": warm["iterations_sum"] / n_test,
+                 "sets_refined": warm["sets_iterated"], "fp64_fallback_sets": warm["fp64_fallback"]},
+        "gain": cold_s / warm_s, "max_l1_warm_vs_cold": diff}))
+
+
+WORKLOADS = {"n16": workload_n16, "uneven": workload_uneven, "n22": workload_n22, "warm": workload_warm}
 
 
 if __name__ == "__main__":

@@ -228,6 +228,10 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
 esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms);
 
+/* Pins the schedule esl_plan_autotune would store (e.g. one found earlier):
+ * schedule 0/1/2 as above with the given restart length, or -1 to forget it. */
+esl_status esl_plan_set_schedule(esl_plan* p, int schedule, int restart);
+
 /* Several plans (distinct networks, e.g. of uneven size) in one call: plan i solves
  * n_sets[i] rate sets d_rates[i] -> d_final[i]; the per-set solves of all plans run
  * concurrently on forked streams (opts->graph = 1: captured once as one HIP graph and

@@ -391,6 +391,10 @@ class solve_plan:
         check(lib().esl_plan_gather_terminal(self.context.handle, self.handle, n_sets, d_final, d_out),
               self.context.handle)
 
+    def set_schedule(self, schedule, restart=32):
+        """Pins a schedule (0 per-set LDS, 1 per-set global, 2 device-wide; -1 forgets)."""
+        check(lib().esl_plan_set_schedule(self.handle, schedule, restart))
+
     def autotune(self, n_sets, d_rates, d_final, **opts):
         """Picks the fastest schedule for this batch (include/esl.h esl_plan_autotune);
         later solves with restart=0 use it.  Returns (schedule, restart, ms)."""

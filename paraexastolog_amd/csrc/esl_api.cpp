@@ -610,6 +610,15 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
 	});
 }
 
+esl_status esl_plan_set_schedule(esl_plan* p, int schedule, int restart)
+{
+	return guarded(nullptr, [&] {
+		require(p != nullptr, ESL_E_ARG, "null plan");
+		require(schedule >= -1 && schedule <= 2, ESL_E_ARG, "schedule must be -1, 0, 1 or 2");
+		esl::plan_set_schedule(p->p, schedule, restart);
+	});
+}
+
 esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms)
 {

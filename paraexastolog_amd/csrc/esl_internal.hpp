@@ -287,6 +287,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
 				   int32_t choice[2], double* best_ms);
 void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final, double* d_out);
+void plan_set_schedule(void* p, int path, int restart);
 // the implementation object behind an esl_ctx handle (defined in esl_api.cpp)
 ctx_t& ctx_impl(esl_ctx* ctx);
 // solution_term / solution_nonterm of one rate set into g.sol (reference layout)

@@ -128,9 +128,11 @@ def test_device_wide_solver_matches_per_set(ctx, pre):
     plan = solve_plan(ctx, t, dx0.ptr)
     assert plan.info()["unknowns"] > 16384
     d16 = DeviceArray(ctx, (B, n), np.float64)
-    plan.solve(B, dr.ptr, d16.ptr, precision=0, tol=1e-13, preconditioner=pre)
+    plan.set_schedule(1, 32)  # one workgroup per set, vectors in global memory
+    plan.solve(B, dr.ptr, d16.ptr, precision=0, tol=1e-13, preconditioner=pre, restart=0)
     d1 = DeviceArray(ctx, (1, n), np.float64)
-    rep = plan.solve(1, dr.offset(2 * net.N * 3), d1.ptr, precision=0, tol=1e-13, preconditioner=pre)
+    plan.set_schedule(2, 32)  # device-wide
+    rep = plan.solve(1, dr.offset(2 * net.N * 3), d1.ptr, precision=0, tol=1e-13, preconditioner=pre, restart=0)
     assert np.abs(d1.download()[0] - d16.download()[3]).sum() <= 1e-11
     if pre == 2:
         assert rep["iterations"] < 500
