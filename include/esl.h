@@ -109,6 +109,17 @@ void esl_graph_destroy(esl_graph* g);
  * descending)], vertices ascending inside an SCC, SCC id = minimum vertex. */
 esl_status esl_graph_reorganize(esl_ctx* ctx, esl_graph* g);
 
+/* The graph stage from stored results instead of recomputing it: the
+ * reordered_vertices (n), sccs_offsets (n_sccs + 1) and terminals_count of an
+ * earlier esl_graph_export of the same table (persistent_data::ordered_vertices,
+ * terminals_offsets ++ nonterminals_offsets, cuda/src/persistent_solution.h:5-24,
+ * loaded by persistent_solution::deserialize, persistent_solution.cu:104-199, and
+ * used by solver(ctx, persisted, rates, state), solver.cu:25-51).  Checked to be a
+ * permutation with ascending blocks; the SCC structure itself is trusted, as the
+ * reference trusts its file.  Heights are not restored (query sizes[6] = 0). */
+esl_status esl_graph_import(esl_ctx* ctx, esl_graph* g, const int32_t* reordered_vertices, const int32_t* sccs_offsets,
+							int64_t n_sccs, int32_t terminals_count);
+
 /* sizes[0] = n (states), [1] = nnz (transitions), [2] = S (SCCs, 0 before
  * reorganize), [3] = T (terminal SCCs), [4] = n_t (states in terminal SCCs),
  * [5] = N (nodes), [6] = max SCC height. */

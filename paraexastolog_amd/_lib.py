@@ -86,6 +86,7 @@ SIGNATURES = [
     ("esl_table_build", C.c_int, [_VP, _VP, _P(C.c_double), _P(_VP)]),
     ("esl_graph_destroy", None, [_VP]),
     ("esl_graph_reorganize", C.c_int, [_VP, _VP]),
+    ("esl_graph_import", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), C.c_int64, C.c_int32]),
     ("esl_graph_query", C.c_int, [_VP, _P(C.c_int64)]),
     ("esl_table_export", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
     ("esl_graph_export", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),

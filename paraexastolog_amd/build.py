@@ -1,4 +1,5 @@
-"""Builds libesl.so (HIP, gfx950) in-tree with hipcc.
+"""Builds libesl.so (HIP, gfx950) in-tree with hipcc, then the command-line
+tool `exastolog` (host C++ over the C ABI) with g++.
 
     python -m paraexastolog_amd.build [--jobs N]
 
@@ -19,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libesl.so")
+CLI = os.path.join(HERE, "exastolog")
 
 SOURCES = ["bnet.cpp", "esl_api.cpp", "table.hip", "graph.hip", "solve.hip", "mlp.hip"]
 
@@ -65,9 +67,26 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
         print(f"built {LIB}")
+        print(f"built {CLI}")
     return LIB
+
+
+def build_cli() -> str:
+    """The drop-in for the reference's command line (host C++, links libesl.so)."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("g++ not found (needed for the exastolog tool)")
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "cli", "exastolog.cpp"), "-L", HERE, "-lesl", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

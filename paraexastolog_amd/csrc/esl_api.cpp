@@ -333,6 +333,16 @@ esl_status esl_graph_reorganize(esl_ctx* ctx, esl_graph* g)
 	});
 }
 
+esl_status esl_graph_import(esl_ctx* ctx, esl_graph* g, const int32_t* reordered_vertices, const int32_t* sccs_offsets,
+							int64_t n_sccs, int32_t terminals_count)
+{
+	return guarded(ctx, [&] {
+		require(ctx && g && reordered_vertices && sccs_offsets, ESL_E_ARG, "null argument");
+		set_device(ctx);
+		esl::graph_import(ctx->c, g->g, reordered_vertices, sccs_offsets, n_sccs, terminals_count);
+	});
+}
+
 esl_status esl_graph_query(const esl_graph* g, int64_t sizes[7])
 {
 	return guarded(nullptr, [&] {

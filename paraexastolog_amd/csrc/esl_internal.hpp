@@ -266,6 +266,7 @@ struct graph_t
 void table_build(ctx_t& c, graph_t& g, const model_t& m);
 void device_copy(ctx_t& c, void* dst, const void* src, size_t bytes);
 void graph_reorganize(ctx_t& c, graph_t& g);
+void graph_import(ctx_t& c, graph_t& g, const int32_t* h_reordered, const int32_t* h_offsets, int64_t S, int64_t T);
 
 struct solve_args_t
 {

@@ -483,4 +483,70 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	g.reorganized = true;
 }
 
+// ---- graph stage results supplied by the caller (a stored .symb file)
+
+// rank[v] = index of v's SCC in the canonical order, scc[v] = its minimum vertex
+// (the first of the SCC's block, vertices being ascending inside a block)
+__global__ __launch_bounds__(gblock) void k_import_rank(int64_t n, int64_t S, const int32_t* __restrict__ offsets,
+														const int32_t* __restrict__ reordered, int32_t* __restrict__ rank,
+														int32_t* __restrict__ scc)
+{
+	const int64_t p = (int64_t)blockIdx.x * gblock + threadIdx.x;
+	if (p >= n)
+		return;
+	int64_t lo = 0, hi = S; // offsets[lo] <= p < offsets[hi]
+	while (hi - lo > 1)
+	{
+		const int64_t mid = (lo + hi) / 2;
+		if (offsets[mid] <= p)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const int32_t v = reordered[p];
+	rank[v] = (int32_t)lo;
+	scc[v] = reordered[offsets[lo]];
+}
+
+void graph_import(ctx_t& c, graph_t& g, const int32_t* h_reordered, const int32_t* h_offsets, int64_t S, int64_t T)
+{
+	const int64_t n = g.n;
+	if (S < 1 || T < 1 || T > S || S > n)
+		throw error(ESL_E_ARG, "graph import: bad SCC counts");
+	if (h_offsets[0] != 0 || h_offsets[S] != n)
+		throw error(ESL_E_ARG, "graph import: offsets must start at 0 and end at the state count");
+	for (int64_t k = 0; k < S; ++k)
+		if (h_offsets[k + 1] <= h_offsets[k])
+			throw error(ESL_E_ARG, "graph import: empty or decreasing SCC block");
+	std::vector<uint8_t> seen(n, 0);
+	for (int64_t k = 0; k < S; ++k)
+		for (int32_t p = h_offsets[k]; p < h_offsets[k + 1]; ++p)
+		{
+			const int32_t v = h_reordered[p];
+			if (v < 0 || v >= n || seen[v])
+				throw error(ESL_E_ARG, "graph import: reordered vertices are not a permutation");
+			seen[v] = 1;
+			if (p > h_offsets[k] && v <= h_reordered[p - 1])
+				throw error(ESL_E_ARG, "graph import: vertices must ascend inside an SCC");
+		}
+	g.reordered.ensure(n);
+	g.offsets.ensure(S + 1);
+	g.rank.ensure(n);
+	g.scc.ensure(n);
+	ESL_HIP(hipMemcpyAsync(g.reordered.get(), h_reordered, n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+	ESL_HIP(hipMemcpyAsync(g.offsets.get(), h_offsets, (S + 1) * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+	k_import_rank<<<ceil_div(n, gblock), gblock, 0, c.stream>>>(n, S, g.offsets.get(), g.reordered.get(),
+																g.rank.get(), g.scc.get());
+	ESL_HIP(hipGetLastError());
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	g.h_offsets.assign(h_offsets, h_offsets + S + 1);
+	g.S = S;
+	g.T = T;
+	g.n_t = h_offsets[T];
+	g.H = 0; // heights are not stored
+	g.stage_ms[1] = 0.0;
+	g.sol = solution_t{};
+	g.reorganized = true;
+}
+
 } // namespace esl
