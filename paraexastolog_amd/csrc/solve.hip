@@ -1483,8 +1483,86 @@ __device__ __forceinline__ void wave_update(const T* __restrict__ V, int nU, int
 	}
 }
 
-template <typename T, int UPT>
-__global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
+// The same two operations on a Krylov basis held in registers: vb[ii][i] is
+// row tid + i * pblock of basis vector ii (MR + 1 vectors, loops unrolled so
+// every index is a compile-time constant; vectors past j are skipped by a
+// uniform branch).
+template <int UPT, int MR>
+__device__ __forceinline__ void reg_dots(const float (&vb)[MR + 1][UPT], int j, const double (&w)[UPT],
+										 double* __restrict__ s_part)
+{
+	const int tid = threadIdx.x, lane = tid & (wave - 1), wv = tid / wave;
+#pragma unroll
+	for (int c0 = 0; c0 <= MR; c0 += dot_chunk)
+	{
+		if (c0 > j)
+			break;
+		double v[dot_chunk];
+#pragma unroll
+		for (int q = 0; q < dot_chunk; ++q)
+		{
+			const int ii = c0 + q <= MR ? c0 + q : MR;
+			double d = 0.0;
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+				d += w[i] * (double)vb[ii][i];
+			v[q] = c0 + q <= j ? d : 0.0;
+		}
+		{
+			const bool hi = lane & 32;
+#pragma unroll
+			for (int k = 0; k < 4; ++k)
+			{
+				const double send = hi ? v[k] : v[k + 4];
+				const double keep = hi ? v[k + 4] : v[k];
+				v[k] = keep + __shfl_xor(send, 32, wave);
+			}
+		}
+		{
+			const bool hi = lane & 16;
+#pragma unroll
+			for (int k = 0; k < 2; ++k)
+			{
+				const double send = hi ? v[k] : v[k + 2];
+				const double keep = hi ? v[k + 2] : v[k];
+				v[k] = keep + __shfl_xor(send, 16, wave);
+			}
+		}
+		{
+			const bool hi = lane & 8;
+			const double send = hi ? v[0] : v[1];
+			const double keep = hi ? v[1] : v[0];
+			v[0] = keep + __shfl_xor(send, 8, wave);
+		}
+		v[0] += __shfl_xor(v[0], 4, wave);
+		v[0] += __shfl_xor(v[0], 2, wave);
+		v[0] += __shfl_xor(v[0], 1, wave);
+		const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+		if ((lane & 7) == 0 && c0 + idx <= j)
+			s_part[(c0 + idx) * pwaves + wv] = v[0];
+	}
+}
+
+template <int UPT, int MR>
+__device__ __forceinline__ void reg_update(const float (&vb)[MR + 1][UPT], int j, const double* __restrict__ c,
+										   double (&w)[UPT])
+{
+#pragma unroll
+	for (int ii = 0; ii <= MR; ++ii)
+	{
+		if (ii > j)
+			break;
+		const double cc = c[ii];
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+			w[i] -= cc * (double)vb[ii][i];
+	}
+}
+
+// MR > 0: the FP32 basis of the thread's rows stays in registers (restart length
+// at most MR); MR = 0: the basis lives in global memory
+template <typename T, int UPT, int MR = 0>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
 													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
 													  int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
 {
@@ -1517,6 +1595,13 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 	const double* __restrict__ ib = S.invd + off;
 	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
 	const int mh = m + 1; // column stride of s_H
+	static_assert(MR == 0 || sizeof(T) == 4, "the register basis is FP32");
+	float vb[MR > 0 ? MR + 1 : 1][UPT];
+#pragma unroll
+	for (int ii = 0; ii < (MR > 0 ? MR + 1 : 1); ++ii)
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+			vb[ii][i] = 0.0f;
 
 	// ---- coupling structure, rates and anchor flags into LDS
 	const int nnz = G.ptr[nU];
@@ -1586,10 +1671,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 
 	int iters = 0, cycles = 0, stalled = 0;
 	double rel = 0.0, rel_prev = 0.0;
-	double corr[UPT]; // last correction of the own rows (undone when an FP32 cycle made things worse)
-#pragma unroll
-	for (int i = 0; i < UPT; ++i)
-		corr[i] = 0.0;
+	double* __restrict__ corr = sc.t + off; // last correction (undone when an FP32 cycle made things worse)
 	for (;;)
 	{
 		// FP64 residual of the own rows
@@ -1625,7 +1707,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 				{
 					const int u = tid + i * pblock;
 					if (u < nU)
-						x[u] -= corr[i];
+						x[u] -= corr[u];
 				}
 				rel = rel_prev;
 			}
@@ -1644,7 +1726,9 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 		{
 			const int u = tid + i * pblock;
 			vj[i] = (T)(ro[i] * inv_beta);
-			if (u < nU)
+			if constexpr (MR > 0)
+				vb[0][i] = (float)vj[i];
+			else if (u < nU)
 				V[u] = vj[i];
 		}
 		if (tid == 0)
@@ -1679,7 +1763,10 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			double w2 = 0.0;
 			for (int pass = 0; pass < 2; ++pass)
 			{
-				wave_dots<T, UPT>(V, nU, j, w, s_part);
+				if constexpr (MR > 0)
+					reg_dots<UPT, MR>(vb, j, w, s_part);
+				else
+					wave_dots<T, UPT>(V, nU, j, w, s_part);
 				__syncthreads();
 				if (tid <= j)
 				{
@@ -1691,7 +1778,10 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 				}
 				__syncthreads();
 				w2 = 0.0;
-				wave_update<T, UPT>(V, nU, j, s_c, w);
+				if constexpr (MR > 0)
+					reg_update<UPT, MR>(vb, j, s_c, w);
+				else
+					wave_update<T, UPT>(V, nU, j, s_c, w);
 #pragma unroll
 				for (int i = 0; i < UPT; ++i)
 					w2 += w[i] * w[i];
@@ -1728,8 +1818,18 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			{
 				const int u = tid + i * pblock;
 				vj[i] = (T)(w[i] * scale);
-				if (u < nU)
-					Vn[u] = vj[i];
+				if constexpr (MR == 0)
+					if (u < nU)
+						Vn[u] = vj[i];
+			}
+			if constexpr (MR > 0)
+			{
+#pragma unroll
+				for (int ii = 1; ii <= MR; ++ii)
+					if (ii == j + 1)
+#pragma unroll
+						for (int i = 0; i < UPT; ++i)
+							vb[ii][i] = (float)vj[i];
 			}
 			++iters;
 			k = j + 1;
@@ -1754,8 +1854,19 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			if (u < nU)
 			{
 				double acc = 0.0;
-				for (int ii = 0; ii < k; ++ii)
-					acc += s_c[ii] * (double)V[(int64_t)ii * nU + u];
+				if constexpr (MR > 0)
+				{
+#pragma unroll
+					for (int ii = 0; ii <= MR; ++ii)
+					{
+						if (ii >= k)
+							break;
+						acc += s_c[ii] * (double)vb[ii][i];
+					}
+				}
+				else
+					for (int ii = 0; ii < k; ++ii)
+						acc += s_c[ii] * (double)V[(int64_t)ii * nU + u];
 				s_t[u] = (P)acc;
 			}
 		}
@@ -1767,8 +1878,9 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 			const int u = tid + i * pblock;
 			if (u < nU)
 			{
-				corr[i] = (double)s_t[u];
-				x[u] += corr[i];
+				const double cu = (double)s_t[u];
+				corr[u] = cu;
+				x[u] += cu;
 			}
 		}
 		__threadfence_block();
@@ -2406,7 +2518,7 @@ struct per_set_scratch_t
 
 // one launch: GMRES of every set, one workgroup per set (no host synchronisation)
 void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs, double* z,
-					const esl_solve_opts& o, per_set_scratch_t& sc, bool allow_lds = true)
+					const esl_solve_opts& o, per_set_scratch_t& sc, bool allow_lds = true, bool allow_reg = true)
 {
 	const int B = S.B;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
@@ -2425,12 +2537,22 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			const bool small = S.nU <= 4 * pblock;
 			if (lp.bytes > 48 * 1024)
 			{
-				const void* fns[4] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
-									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8> };
+				const void* fns[5] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
+									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
+									   (const void*)k_gmres_lds<float, 4, 32> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
-			if (o.precision == 1)
+			if (o.precision == 1 && small && m <= 32 && allow_reg)
+			{
+				// FP32 basis rows in registers: no basis traffic at all
+				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				timed_launch(c, "k_gmres_reg<float>", by, [&] {
+					k_gmres_lds<float, 4, 32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																				 o.max_iters, po, lp, o.inner_rtol);
+				});
+			}
+			else if (o.precision == 1)
 			{
 				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
 				timed_launch(c, "k_gmres_lds<float>", by, [&] {
@@ -2535,10 +2657,12 @@ struct plan_t
 	struct tune_t
 	{
 		bool set = false;
-		int path = 0; // 0 per-set (LDS-resident when it fits), 1 per-set from global memory, 2 device-wide
+		int path = 0; // 0 per-set (LDS-resident, basis in registers when they fit), 1 per-set from global
+					  // memory, 2 device-wide, 3 per-set LDS-resident with the basis in global memory
 		int m = 32;
 	} tune;
 	bool allow_lds = true; // the schedule of the run being enqueued
+	bool allow_reg = true;
 	double ms[3] = { 0, 0, 0 };
 	// captured numeric sequence and the arguments it was captured with
 	struct gkey_t
@@ -2551,12 +2675,13 @@ struct plan_t
 		double tol = 0, inner_rtol = 0;
 		int max_iters = 0;
 		int64_t epoch = 0;
-		bool allow_lds = true;
+		bool allow_lds = true, allow_reg = true;
 		bool operator==(const gkey_t& o) const
 		{
 			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
 				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && inner_rtol == o.inner_rtol &&
-				   max_iters == o.max_iters && epoch == o.epoch && allow_lds == o.allow_lds;
+				   max_iters == o.max_iters && epoch == o.epoch && allow_lds == o.allow_lds &&
+				   allow_reg == o.allow_reg;
 		}
 	};
 	gkey_t gkey{};
@@ -2902,7 +3027,8 @@ void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_f
 {
 	plan_enqueue_setup(c, P, B, d_rates, o);
 	if (P.nU)
-		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr, P.allow_lds);
+		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr, P.allow_lds,
+					   P.allow_reg);
 	plan_enqueue_post(c, P, B, d_rates, d_final);
 }
 
@@ -2953,6 +3079,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	if (tuned)
 		o.restart = P.tune.m;
 	P.allow_lds = !(tuned && P.tune.path == 1);
+	P.allow_reg = !(tuned && P.tune.path == 3);
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
 	const bool per_set = P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B));
 	rep.terminals_reached = P.Tr;
@@ -2965,8 +3092,9 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		const bool use_graph = o.graph != 0;
 		if (use_graph)
 		{
-			const plan_t::gkey_t key{ B, d_rates, d_final,	   o.warm_pi, o.precision, o.preconditioner, m,
-									  o.tol, o.inner_rtol, o.max_iters, P.epoch,   P.allow_lds };
+			const plan_t::gkey_t key{ B,		   d_rates,	  d_final,	   o.warm_pi,  o.precision,
+									  o.preconditioner, m,		  o.tol,	   o.inner_rtol, o.max_iters,
+									  P.epoch,		   P.allow_lds, P.allow_reg };
 			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
 			{
 				P.drop_graph();
@@ -3129,6 +3257,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		reps[i] = esl_solve_report{};
 		const bool tuned = o_in.restart == 0 && P.tune.set;
 		P.allow_lds = !(tuned && P.tune.path == 1);
+		P.allow_reg = !(tuned && P.tune.path == 3);
 		po[i].restart = std::min(std::max(tuned ? P.tune.m : (o.restart > 0 ? o.restart : 32), 2), pm_max);
 		if (P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B[i])))
 			conc.push_back(i);
@@ -3174,7 +3303,8 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 			key.fin.push_back(d_final[i]);
 			key.epochs.push_back(static_cast<plan_t*>(plans[i])->epoch);
 			key.ms.push_back(po[i].restart);
-			key.lds.push_back(static_cast<plan_t*>(plans[i])->allow_lds ? 1 : 0);
+			key.lds.push_back((static_cast<plan_t*>(plans[i])->allow_lds ? 1 : 0) +
+							  (static_cast<plan_t*>(plans[i])->allow_reg ? 2 : 0));
 		}
 		key.precision = o.precision;
 		key.preconditioner = o.preconditioner;
@@ -3311,6 +3441,8 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 			cands.push_back({ 0, m });
 			if (ilu && P.nU <= lds_max_unknowns) // otherwise path 1 is path 0
 				cands.push_back({ 1, m });
+			if (ilu && o.precision == 1 && P.nU <= 4 * pblock && m <= 32) // otherwise path 3 is path 0
+				cands.push_back({ 3, m });
 			cands.push_back({ 2, m });
 		}
 	const plan_t::tune_t saved = P.tune;
