@@ -1561,10 +1561,11 @@ __device__ __forceinline__ void reg_update(const float (&vb)[MR + 1][UPT], int j
 
 // MR > 0: the FP32 basis of the thread's rows stays in registers (restart length
 // at most MR); MR = 0: the basis lives in global memory
-template <typename T, int UPT, int MR = 0>
-__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
-													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
-													  int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+template <typename T, int UPT, int MR>
+__device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, int n_seg, const double* __restrict__ rhs,
+											   double* __restrict__ zio, const pgm_scratch_t& sc, int m, double tol,
+											   int max_iters, const pgm_out_t& out, const lds_plan_t& lp,
+											   double inner_rtol)
 {
 	using P = T; // precision of the LDS vectors (preconditioned vector, inverse pivots)
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -1902,6 +1903,24 @@ __global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) voi
 	}
 }
 
+
+// waves_per_eu: the register budget of each variant (2 waves per SIMD leave 256
+// VGPRs per lane for a 33-vector register basis, 3 leave 168 for 17 vectors)
+template <typename T, int UPT, int MR = 0>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_lds(
+	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
+	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+{
+	gmres_lds_body<T, UPT, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+}
+
+template <int MR>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(3))) void k_gmres_reg3(
+	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
+	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+{
+	gmres_lds_body<float, 4, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+}
 
 // ============================================================================
 // device-wide GMRES kernels (nbr_t coupling lists, iterate x unpreconditioned)
@@ -2537,19 +2556,24 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			const bool small = S.nU <= 4 * pblock;
 			if (lp.bytes > 48 * 1024)
 			{
-				const void* fns[5] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
+				const void* fns[6] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
 									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
-									   (const void*)k_gmres_lds<float, 4, 32> };
+									   (const void*)k_gmres_lds<float, 4, 32>, (const void*)k_gmres_reg3<16> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
 			if (o.precision == 1 && small && m <= 32 && allow_reg)
 			{
-				// FP32 basis rows in registers: no basis traffic at all
+				// FP32 basis rows in registers: no basis traffic at all (17 vectors
+				// leave room for 3 workgroups per CU, 33 for 2)
 				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
 				timed_launch(c, "k_gmres_reg<float>", by, [&] {
-					k_gmres_lds<float, 4, 32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																				 o.max_iters, po, lp, o.inner_rtol);
+					if (m <= 16)
+						k_gmres_reg3<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			 o.max_iters, po, lp, o.inner_rtol);
+					else
+						k_gmres_lds<float, 4, 32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																					 o.max_iters, po, lp, o.inner_rtol);
 				});
 			}
 			else if (o.precision == 1)
