@@ -705,6 +705,96 @@ __global__ __launch_bounds__(sblock) void k_anchor_pick(int64_t nUt, const int32
 		anchor[(int64_t)b * n_seg + k] = s_i[0];
 }
 
+// The anchor choice in one launch when the terminal unknowns fit in LDS: one
+// workgroup per rate set runs the same damped Jacobi sweeps as k_anchor_sweep
+// (in-neighbours from the coupling lists, summed in their order) and the pick
+// of k_anchor_pick.  Dynamic LDS: p, p', D (nUt doubles each), row pointers
+// (nUt + 1 int32), neighbour (uint16) and rate code (uint8) per coupling.
+__global__ __launch_bounds__(sblock) void k_anchor_lds(sys_t S, const int32_t* __restrict__ gptr,
+													   const int32_t* __restrict__ gnu, const uint8_t* __restrict__ gnr,
+													   const int32_t* __restrict__ seg_off, int n_seg, int sweeps,
+													   int32_t* __restrict__ anchor)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	__shared__ double s_rates[64];
+	__shared__ double s_v[sblock];
+	__shared__ int32_t s_i[sblock];
+	const int nUt = (int)S.nUt;
+	const int b = blockIdx.x;
+	const int tid = threadIdx.x;
+	double* p = (double*)smem;
+	double* pn = p + nUt;
+	double* d = pn + nUt;
+	int32_t* ptr = (int32_t*)(d + nUt);
+	const int nnz = gptr[nUt];
+	uint16_t* nu = (uint16_t*)(ptr + nUt + 1);
+	uint8_t* nc = (uint8_t*)(nu + nnz);
+	for (int i = tid; i < 2 * S.N; i += sblock)
+		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
+	__syncthreads();
+	for (int u = tid; u < nUt; u += sblock)
+	{
+		const uint32_t st = (uint32_t)S.ulist[u];
+		p[u] = 1.0;
+		d[u] = out_rate_s(st, S.mask[st], s_rates);
+		ptr[u] = gptr[u];
+	}
+	if (tid == 0)
+		ptr[nUt] = nnz;
+	for (int k = tid; k < nnz; k += sblock)
+	{
+		nu[k] = (uint16_t)gnu[k];
+		nc[k] = gnr[k];
+	}
+	__syncthreads();
+	for (int it = 0; it < sweeps; ++it)
+	{
+		for (int u = tid; u < nUt; u += sblock)
+		{
+			double inflow = 0.0;
+			for (int k = ptr[u]; k < ptr[u + 1]; ++k)
+				inflow += s_rates[nc[k] & 63] * p[nu[k]];
+			pn[u] = 0.5 * p[u] + 0.5 * inflow / d[u];
+		}
+		__syncthreads();
+		double* t = p;
+		p = pn;
+		pn = t;
+	}
+	for (int k = 0; k < n_seg; ++k)
+	{
+		const int32_t a = seg_off[k], e = seg_off[k + 1];
+		double best = -1.0;
+		int32_t bi = a;
+		for (int32_t u = a + tid; u < e; u += sblock)
+			if (p[u] > best)
+			{
+				best = p[u];
+				bi = u;
+			}
+		s_v[tid] = best;
+		s_i[tid] = bi;
+		__syncthreads();
+		for (int w = sblock / 2; w > 0; w >>= 1)
+		{
+			if (tid < w)
+			{
+				const double v2 = s_v[tid + w];
+				const int32_t i2 = s_i[tid + w];
+				if (v2 > s_v[tid] || (v2 == s_v[tid] && i2 < s_i[tid]))
+				{
+					s_v[tid] = v2;
+					s_i[tid] = i2;
+				}
+			}
+			__syncthreads();
+		}
+		if (tid == 0)
+			anchor[(int64_t)b * n_seg + k] = s_i[0];
+		__syncthreads();
+	}
+}
+
 // right-hand sides: anchors -> 1, other terminal unknowns -> 0, ambiguous -> x0
 __global__ __launch_bounds__(sblock) void k_rhs(sys_t S, int n_seg, const double* __restrict__ x0,
 												double* __restrict__ rhs)
@@ -2676,6 +2766,7 @@ struct plan_t
 	dbuf<int32_t> anchor;
 	per_set_scratch_t scr;
 	int anchor_sweeps = 32;
+	size_t anchor_lds = 0; // dynamic LDS of k_anchor_lds, 0 when the terminal unknowns do not fit
 	int64_t epoch = 0; // bumped whenever a per-run buffer is reallocated
 	// schedule chosen by plan_autotune, used when a solve asks for restart 0
 	struct tune_t
@@ -2943,10 +3034,27 @@ void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 	}
 	// ---- coupling structure of the per-set solver
 	P.nbr.ready = false;
+	P.anchor_lds = 0;
 	if (P.nU)
 	{
 		sys_t S = P.sys(1, nullptr);
 		build_nbr(c, S, P.nbr);
+		if (P.nUt > 0 && P.nUt <= 4096)
+		{
+			int32_t nnz_t = 0;
+			ESL_HIP(hipMemcpyAsync(&nnz_t, P.nbr.ptr.get() + P.nUt, sizeof(int32_t), hipMemcpyDeviceToHost,
+								   c.stream));
+			ESL_HIP(hipStreamSynchronize(c.stream));
+			const size_t bytes = 24 * (size_t)P.nUt + 4 * (size_t)(P.nUt + 1) + 3 * (size_t)nnz_t + 16;
+			const size_t static_bytes = 64 * 8 + sblock * 12 + 256;
+			if (bytes + static_bytes <= (size_t)std::min(c.lds_per_block, 160 * 1024))
+			{
+				P.anchor_lds = bytes;
+				if (bytes > 48 * 1024)
+					ESL_HIP(hipFuncSetAttribute((const void*)k_anchor_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+												(int)bytes));
+			}
+		}
 	}
 	P.ms[2] = tm_.lap();
 }
@@ -2987,6 +3095,15 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 		if (warm)
 			// the prediction picks the anchors (no Jacobi sweeps)
 			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, P.d_tpos.get(), o.warm_pi, P.p0.get());
+		else if (P.anchor_lds)
+		{
+			// sweeps and pick in one launch, the terminal unknowns in LDS
+			timed_launch(c, "k_anchor_lds", 8.0 * B * nUt, [&] {
+				k_anchor_lds<<<B, sblock, P.anchor_lds, c.stream>>>(S, P.nbr.ptr.get(), P.nbr.nu.get(),
+																	 P.nbr.nr.get(), P.d_seg_off.get(), n_seg,
+																	 P.anchor_sweeps, P.anchor.get());
+			});
+		}
 		else
 		{
 			k_fill_value<<<ceil_div((int64_t)B * nUt, sblock), sblock, 0, c.stream>>>((int64_t)B * nUt, 1.0,
@@ -2997,8 +3114,9 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 				std::swap(P.p0, P.p1);
 			}
 		}
-		k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.p0.get(), n_seg,
-																 P.anchor.get());
+		if (warm || !P.anchor_lds)
+			k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.p0.get(), n_seg,
+																	 P.anchor.get());
 	}
 	if (nU)
 	{
