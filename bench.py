@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="residual reduction an FP32 cycle aims for (library default when omitted)")
     ap.add_argument("--autotune", type=int, default=1, help="pick the schedule with esl_plan_autotune first")
+    ap.add_argument("--schedule", type=int, default=None,
+                    help="pin a schedule instead of autotuning (0 per-set, 1 per-set global, 2 device-wide, "
+                         "3 per-set LDS with the basis in global memory)")
     ap.add_argument("--workload", default="n20", choices=["n20", "n16", "uneven", "n22", "warm"],
                     help="n20 (default) is the headline metric; the others print their own JSON line")
     ap.add_argument("--epochs", type=int, default=30, help="training epochs of the warm workload")
@@ -203,7 +206,12 @@ def main():
     tol = 1e-12
     schedule = None
     restart = 32
-    if a.autotune:
+    if a.schedule is not None:
+        check(L.esl_plan_set_schedule(pl, a.schedule, 32))
+        schedule = {"path": ["per-set", "per-set global", "device-wide", "per-set LDS, basis in memory"][a.schedule],
+                    "restart": 32, "pinned": True}
+        restart = 0
+    elif a.autotune:
         o = solve_opts(precision=a.precision, tol=tol, inner_rtol=a.inner_rtol)
         o.graph = a.graph
         ch = (C.c_int32 * 2)()

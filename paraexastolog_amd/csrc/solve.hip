@@ -1712,12 +1712,12 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 	for (int i = tid; i < 2 * S.N; i += pblock)
 		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
 	for (int w = tid; w < (nU + 31) / 32; w += pblock)
-	{
-		uint32_t bits = 0;
-		for (int i = 0; i < 32 && w * 32 + i < nU; ++i)
-			bits |= is_anchor(S, n_seg, b, w * 32 + i) ? (1u << i) : 0u;
-		s_anc[w] = bits;
-	}
+		s_anc[w] = 0u;
+	__syncthreads();
+	// one row per thread (independent loads), bits set with LDS atomics
+	for (int u = tid; u < nU; u += pblock)
+		if (is_anchor(S, n_seg, b, u))
+			atomicOr(&s_anc[u >> 5], 1u << (u & 31));
 	double dg[UPT]; // D' of the own rows
 #pragma unroll
 	for (int i = 0; i < UPT; ++i)
