@@ -1573,10 +1573,9 @@ __device__ __forceinline__ void wave_update(const T* __restrict__ V, int nU, int
 	}
 }
 
-// The same two operations on a Krylov basis held in registers: vb[ii][i] is
-// row tid + i * pblock of basis vector ii (MR + 1 vectors, loops unrolled so
-// every index is a compile-time constant; vectors past j are skipped by a
-// uniform branch).
+// The same two operations on a Krylov basis held as a per-thread array:
+// vb[ii][i] is row tid + i * pblock of basis vector ii (MR + 1 vectors; vectors
+// past j are skipped by a uniform branch).
 template <int UPT, int MR>
 __device__ __forceinline__ void reg_dots(const float (&vb)[MR + 1][UPT], int j, const double (&w)[UPT],
 										 double* __restrict__ s_part)
@@ -1649,8 +1648,8 @@ __device__ __forceinline__ void reg_update(const float (&vb)[MR + 1][UPT], int j
 	}
 }
 
-// MR > 0: the FP32 basis of the thread's rows stays in registers (restart length
-// at most MR); MR = 0: the basis lives in global memory
+// MR > 0: the FP32 basis of the thread's rows is a private per-thread array
+// (restart length at most MR); MR = 0: the basis lives in global memory
 template <typename T, int UPT, int MR>
 __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, int n_seg, const double* __restrict__ rhs,
 											   double* __restrict__ zio, const pgm_scratch_t& sc, int m, double tol,
@@ -1686,7 +1685,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 	const double* __restrict__ ib = S.invd + off;
 	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
 	const int mh = m + 1; // column stride of s_H
-	static_assert(MR == 0 || sizeof(T) == 4, "the register basis is FP32");
+	static_assert(MR == 0 || sizeof(T) == 4, "the private basis is FP32");
 	float vb[MR > 0 ? MR + 1 : 1][UPT];
 #pragma unroll
 	for (int ii = 0; ii < (MR > 0 ? MR + 1 : 1); ++ii)
@@ -1994,18 +1993,23 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 }
 
 
-// waves_per_eu: the register budget of each variant (2 waves per SIMD leave 256
-// VGPRs per lane for a 33-vector register basis, 3 leave 168 for 17 vectors)
-template <typename T, int UPT, int MR = 0>
-__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_lds(
-	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
-	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+// The variants differ in where the FP32 Krylov basis of a set lives: in global
+// memory (MR = 0), or as a per-thread array of MR + 1 vectors of its own rows
+// (MR > 0).  The compiler keeps such an array in lane-private (scratch) memory
+// once registers run out; limited to 128 VGPRs (4 waves per SIMD, 4 workgroups
+// per CU, the LDS limit as well) this measured fastest on MI355X: 2.56 ms for
+// 1024 sets of 1022 unknowns against 2.88 ms with 256 VGPRs (2 workgroups per
+// CU) and 3.97 ms with the basis in global memory.
+template <typename T, int UPT>
+__global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
+													  double* __restrict__ zio, pgm_scratch_t sc, int m, double tol,
+													  int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
 {
-	gmres_lds_body<T, UPT, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+	gmres_lds_body<T, UPT, 0>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
 }
 
 template <int MR>
-__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(3))) void k_gmres_reg3(
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(4))) void k_gmres_priv(
 	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
 	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
 {
@@ -2648,22 +2652,21 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			{
 				const void* fns[6] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
 									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
-									   (const void*)k_gmres_lds<float, 4, 32>, (const void*)k_gmres_reg3<16> };
+									   (const void*)k_gmres_priv<32>, (const void*)k_gmres_priv<16> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
 			if (o.precision == 1 && small && m <= 32 && allow_reg)
 			{
-				// FP32 basis rows in registers: no basis traffic at all (17 vectors
-				// leave room for 3 workgroups per CU, 33 for 2)
+				// the FP32 basis of each thread's rows as a private array
 				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
-				timed_launch(c, "k_gmres_reg<float>", by, [&] {
+				timed_launch(c, "k_gmres_priv<float>", by, [&] {
 					if (m <= 16)
-						k_gmres_reg3<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+						k_gmres_priv<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
 																			 o.max_iters, po, lp, o.inner_rtol);
 					else
-						k_gmres_lds<float, 4, 32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																					 o.max_iters, po, lp, o.inner_rtol);
+						k_gmres_priv<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			 o.max_iters, po, lp, o.inner_rtol);
 				});
 			}
 			else if (o.precision == 1)
@@ -2772,7 +2775,7 @@ struct plan_t
 	struct tune_t
 	{
 		bool set = false;
-		int path = 0; // 0 per-set (LDS-resident, basis in registers when they fit), 1 per-set from global
+		int path = 0; // 0 per-set (LDS-resident, private per-thread basis when it fits), 1 per-set from global
 					  // memory, 2 device-wide, 3 per-set LDS-resident with the basis in global memory
 		int m = 32;
 	} tune;
