@@ -420,11 +420,13 @@ __global__ __launch_bounds__(wave) void k_reduce_parts(const double* __restrict_
 template <typename T, int KMAX>
 __global__ __launch_bounds__(sblock) void k_dots(int64_t nU, int B, int K, const T* __restrict__ W,
 												 const T* __restrict__ V, int64_t vstride,
-												 double* __restrict__ part)
+												 double* __restrict__ part, const int32_t* __restrict__ live)
 {
 	__shared__ double s_red[sblock / wave + 1];
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
+	if (!live[b]) // the whole block belongs to one set: uniform exit before any barrier
+		return;
 	double w = 0.0;
 	if (u < nU)
 		w = (double)W[(int64_t)b * nU + u];
@@ -444,11 +446,13 @@ template <typename T>
 __global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __restrict__ W,
 														const T* __restrict__ V, int64_t vstride,
 														const double* __restrict__ c, int64_t cstride,
-														double* __restrict__ part)
+														double* __restrict__ part, const int32_t* __restrict__ live)
 {
 	__shared__ double s_red[sblock / wave + 1];
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
+	if (!live[b]) // uniform per block
+		return;
 	double w2 = 0.0;
 	if (u < nU)
 	{
@@ -468,11 +472,12 @@ __global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __
 // V_next = W / h  (h = 0 leaves V_next = 0)
 template <typename T>
 __global__ __launch_bounds__(sblock) void k_scale(int64_t nU, const T* __restrict__ W,
-												  const double* __restrict__ hnorm, T* __restrict__ Vn)
+												  const double* __restrict__ hnorm, T* __restrict__ Vn,
+												  const int32_t* __restrict__ live)
 {
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
-	if (u >= nU)
+	if (u >= nU || !live[b])
 		return;
 	const double h = hnorm[b];
 	const double inv = h > 0.0 ? 1.0 / h : 0.0;
@@ -493,6 +498,9 @@ struct gm_state_t
 	double* bnorm; // [B]
 	double* res;   // [B] current residual estimate / |rhs|
 	int32_t* live; // [B] 1 while the set iterates in this cycle
+	int32_t* act;  // [B] 1 when the set started this cycle above the tolerance
+	int32_t* kb;   // [B] Arnoldi steps the set took in this cycle
+	double* res0;  // [B] relative residual at the start of the cycle
 };
 
 // start of a cycle: beta = |r|, g = (beta, 0..), live = beta > tol*|rhs|
@@ -511,13 +519,18 @@ __global__ void k_gm_start(int B, int m, gm_state_t st, const double* __restrict
 	st.g[(int64_t)b * (m + 1)] = beta;
 	const double rel = bn > 0.0 ? beta / bn : beta;
 	st.res[b] = rel;
+	st.res0[b] = rel;
 	const int live = rel > tol ? 1 : 0;
 	st.live[b] = live;
+	st.act[b] = live;
+	st.kb[b] = 0;
 	vscale[b] = live ? beta : 0.0; // V_0 = r / beta (0 when the set is done)
 }
 
-// H[:, j] = c1 + c2, H[j+1, j] = |w|, Givens update, residual estimate
-__global__ void k_gm_givens(int B, int m, int j, gm_state_t st)
+// H[:, j] = c1 + c2, H[j+1, j] = |w|, Givens update, residual estimate; a set
+// stops iterating (and drops out of the Arnoldi kernels) once its estimate
+// reaches max(tol_half, rtol * its residual at the start of the cycle)
+__global__ void k_gm_givens(int B, int m, int j, gm_state_t st, double tol_half, double rtol)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
 	if (b >= B)
@@ -537,6 +550,7 @@ __global__ void k_gm_givens(int B, int m, int j, gm_state_t st)
 		st.hn[b] = 0.0;
 		return;
 	}
+	st.kb[b] = j + 1;
 	for (int i = 0; i <= j; ++i)
 		col[i] = st.c1[(int64_t)b * (m + 1) + i] + st.c2[(int64_t)b * (m + 1) + i];
 	const double hn = sqrt(st.hn[b]);
@@ -565,7 +579,7 @@ __global__ void k_gm_givens(int B, int m, int j, gm_state_t st)
 	const double bn = st.bnorm[b];
 	const double rel = fabs(g[j + 1]) / (bn > 0.0 ? bn : 1.0);
 	st.res[b] = rel;
-	if (!(hn > 0.0) || !(rnorm > 0.0))
+	if (!(hn > 0.0) || !(rnorm > 0.0) || rel <= fmax(tol_half, rtol * st.res0[b]))
 		st.live[b] = 0;
 }
 
@@ -578,6 +592,7 @@ __global__ void k_gm_solve_y(int B, int m, int k, gm_state_t st)
 	const double* H = st.H + (int64_t)b * (m + 1) * m;
 	const double* g = st.g + (int64_t)b * (m + 1);
 	double* y = st.c1 + (int64_t)b * (m + 1);
+	k = min(k, st.kb[b]); // the set's own steps in this cycle
 	for (int i = k - 1; i >= 0; --i)
 	{
 		double s = g[i];
@@ -593,11 +608,15 @@ __global__ void k_gm_solve_y(int B, int m, int k, gm_state_t st)
 // z += sum_i y_i V_i
 template <typename T, typename Z>
 __global__ __launch_bounds__(sblock) void k_combine(int64_t nU, int k, const T* __restrict__ V, int64_t vstride,
-													const double* __restrict__ y, int m, Z* __restrict__ z)
+													const double* __restrict__ y, int m, Z* __restrict__ z,
+													const int32_t* __restrict__ kb)
 {
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
 	if (u >= nU)
+		return;
+	k = min(k, kb[b]);
+	if (k == 0)
 		return;
 	double acc = (double)z[(int64_t)b * nU + u];
 	const double* yb = y + (int64_t)b * (m + 1);
@@ -2026,11 +2045,11 @@ __device__ __forceinline__ const double* set_rates(const sys_t& S, int b)
 
 template <typename X>
 __global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg, const double* __restrict__ tin,
-													 X* __restrict__ W)
+													 X* __restrict__ W, const int32_t* __restrict__ live)
 {
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
-	if (u >= S.nU)
+	if (u >= S.nU || !live[b])
 		return;
 	const int64_t off = (int64_t)b * S.nU;
 	W[off + u] = (X)row_apply<double>(S, G, n_seg, b, u, tin + off, set_rates(S, b));
@@ -2064,11 +2083,11 @@ __global__ __launch_bounds__(sblock) void k_mk_residual(sys_t S, nbr_t G, int n_
 }
 
 template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_jacobi(sys_t S, X* __restrict__ v)
+__global__ __launch_bounds__(sblock) void k_mk_jacobi(sys_t S, X* __restrict__ v, const int32_t* __restrict__ live)
 {
 	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
-	if (u >= S.nU)
+	if (u >= S.nU || !live[b])
 		return;
 	const int64_t i = (int64_t)b * S.nU + u;
 	v[i] = (X)((double)v[i] * S.invd[i]);
@@ -2098,11 +2117,12 @@ __global__ __launch_bounds__(sblock) void k_mk_ilu_pivot(sys_t S, nbr_t G, int n
 
 template <typename X>
 __global__ __launch_bounds__(sblock) void k_mk_ilu_fwd(sys_t S, nbr_t G, int n_seg, int l,
-													   const double* __restrict__ ud, X* __restrict__ v)
+													   const double* __restrict__ ud, X* __restrict__ v,
+													   const int32_t* __restrict__ live)
 {
 	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
-	if (q >= G.L.off[l + 1])
+	if (q >= G.L.off[l + 1] || !live[b])
 		return;
 	const int32_t u = G.lvl[q];
 	if (is_anchor(S, n_seg, b, u))
@@ -2120,11 +2140,12 @@ __global__ __launch_bounds__(sblock) void k_mk_ilu_fwd(sys_t S, nbr_t G, int n_s
 
 template <typename X>
 __global__ __launch_bounds__(sblock) void k_mk_ilu_bwd(sys_t S, nbr_t G, int n_seg, int l,
-													   const double* __restrict__ ud, X* __restrict__ v)
+													   const double* __restrict__ ud, X* __restrict__ v,
+													   const int32_t* __restrict__ live)
 {
 	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
 	const int b = blockIdx.y;
-	if (q >= G.L.off[l + 1])
+	if (q >= G.L.off[l + 1] || !live[b])
 		return;
 	const int32_t u = G.lvl[q];
 	if (is_anchor(S, n_seg, b, u))
@@ -2135,6 +2156,18 @@ __global__ __launch_bounds__(sblock) void k_mk_ilu_bwd(sys_t S, nbr_t G, int n_s
 	for (int32_t k = G.mid[u]; k < G.ptr[u + 1]; ++k)
 		z += r[G.nr[k] & 63] * (double)v[off + G.nu[k]];
 	v[off + u] = (X)(z / ud[off + u]);
+}
+
+// b[set][u] = a[set][u] for the sets still iterating (grid: unknowns x sets)
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_copy_live(int64_t nU, const T* __restrict__ a, double* __restrict__ out,
+													  const int32_t* __restrict__ live)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= nU || !live[b])
+		return;
+	out[(int64_t)b * nU + u] = (double)a[(int64_t)b * nU + u];
 }
 
 template <typename T>
@@ -2386,10 +2419,11 @@ struct gmres_t
 	bool ilu;
 	dbuf<T> V, W;
 	dbuf<double> t; // preconditioned vector, FP64
-	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r, ud;
-	dbuf<int32_t> live;
+	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r, ud, res0;
+	dbuf<int32_t> live, act, kb;
 	gm_state_t st{};
 	std::vector<double> h_res;
+	std::vector<int32_t> h_live;
 	int iters = 0, cycles = 0;
 	double final_res = 0.0;
 
@@ -2414,11 +2448,15 @@ struct gmres_t
 		res.ensure(B);
 		vscale.ensure(B);
 		live.ensure(B);
+		act.ensure(B);
+		kb.ensure(B);
+		res0.ensure(B);
 		const int nblk = ceil_div(nU, sblock);
 		part.ensure((size_t)B * nblk * (m + 1) + 16);
 		rr.ensure((size_t)B * 2);
-		st = gm_state_t{ H.get(), cs.get(), sn.get(), g.get(), c1.get(), c2.get(), hn.get(),
-						 beta.get(), bnorm.get(), res.get(), live.get() };
+		st = gm_state_t{ H.get(),	 cs.get(),	  sn.get(),	  g.get(),	 c1.get(),	c2.get(),
+						 hn.get(),	 beta.get(), bnorm.get(), res.get(), live.get(), act.get(),
+						 kb.get(),	 res0.get() };
 		h_res.resize(B);
 	}
 
@@ -2432,27 +2470,39 @@ struct gmres_t
 	}
 
 	template <typename X>
-	void precondition(X* v) // in place: v <- P^{-1} v
+	void precondition(X* v, const int32_t* flags) // in place: v <- P^{-1} v for the sets with flags[b] != 0
 	{
 		const nbr_t G = NB.view();
 		const int B = S.B;
 		if (!ilu)
 		{
-			k_mk_jacobi<X><<<dim3(ceil_div(S.nU, sblock), B), sblock, 0, c.stream>>>(S, v);
+			k_mk_jacobi<X><<<dim3(ceil_div(S.nU, sblock), B), sblock, 0, c.stream>>>(S, v, flags);
 			return;
 		}
 		for (int l = 0; l < G.L.n; ++l)
 		{
 			const int cnt = G.L.off[l + 1] - G.L.off[l];
 			if (cnt)
-				k_mk_ilu_fwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v);
+				k_mk_ilu_fwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v,
+																						  flags);
 		}
 		for (int l = G.L.n - 1; l >= 0; --l)
 		{
 			const int cnt = G.L.off[l + 1] - G.L.off[l];
 			if (cnt)
-				k_mk_ilu_bwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v);
+				k_mk_ilu_bwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v,
+																						  flags);
 		}
+	}
+
+	// true while any set still iterates in this cycle
+	bool any_live()
+	{
+		read_host(c, h_live, live.get(), (size_t)S.B);
+		for (int32_t v : h_live)
+			if (v)
+				return true;
+		return false;
 	}
 
 	// x: FP64 iterate (in/out).  Returns false when an FP32 cycle failed to
@@ -2495,7 +2545,6 @@ struct gmres_t
 				return false;
 			}
 			prev = final_res;
-			const double inner_target = std::max(0.5 * tol, sizeof(T) == 4 ? inner_rtol * final_res : 0.0);
 			if (iters >= max_iters)
 				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
 											   std::to_string(final_res) + " after " + std::to_string(iters) +
@@ -2506,40 +2555,45 @@ struct gmres_t
 			for (int j = 0; j < m; ++j)
 			{
 				T* Vj = V.get() + (int64_t)j * vstride;
-				k_copy<T><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, Vj, t.get());
-				precondition<double>(t.get());
+				k_copy_live<T><<<grid, sblock, 0, c.stream>>>(nU, Vj, t.get(), live.get());
+				precondition<double>(t.get(), live.get());
 				// algorithmic bytes: t read, W written, coupling structure and 1/D' read
 				timed_launch(c, "k_mk_apply", (double)B * nU * (8.0 + sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz,
-							 [&] { k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get()); });
+							 [&] {
+								 k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get(), live.get());
+							 });
 				// classical Gram-Schmidt, twice
 				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
 				timed_launch(c, "k_dots", dbytes, [&] {
-					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get(),
+																 live.get());
 				});
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
 				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
 					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
-																	 m + 1, part.get());
+																	 m + 1, part.get(), live.get());
 				});
-				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get());
+				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get(),
+															 live.get());
 				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
 				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
-																 m + 1, part.get());
+																 m + 1, part.get(), live.get());
 				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
-				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st);
-				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride);
+				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st, 0.5 * tol, sizeof(T) == 4 ? inner_rtol : 0.0);
+				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride,
+														  live.get());
 				ESL_HIP(hipGetLastError());
 				++iters;
 				k = j + 1;
-				const double est = max_res();
-				if (est <= inner_target || iters >= max_iters)
+				if (!any_live() || iters >= max_iters)
 					break;
 			}
 			// x += P^{-1} (sum_i y_i V_i) in FP64
 			k_gm_solve_y<<<sb, 64, 0, c.stream>>>(B, m, k, st);
 			k_zero<double><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get());
-			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, r.get());
-			precondition<double>(r.get());
+			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, r.get(),
+															   kb.get());
+			precondition<double>(r.get(), act.get());
 			if (sizeof(T) == 4) // keep the correction in t until the next residual has been checked
 				k_axpy1_keep<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x, t.get());
 			else
