@@ -2027,6 +2027,16 @@ __global__ __launch_bounds__(pblock) void k_gmres_lds(sys_t S, nbr_t G, int n_se
 	gmres_lds_body<T, UPT, 0>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
 }
 
+// 8 rows per thread (up to 2048 unknowns): LDS already limits these to 2
+// workgroups per CU
+template <int MR>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_priv8(
+	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
+	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+{
+	gmres_lds_body<float, 8, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+}
+
 template <int MR>
 __global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(4))) void k_gmres_priv(
 	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
@@ -2704,13 +2714,26 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			const bool small = S.nU <= 4 * pblock;
 			if (lp.bytes > 48 * 1024)
 			{
-				const void* fns[6] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
+				const void* fns[8] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
 									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
-									   (const void*)k_gmres_priv<32>, (const void*)k_gmres_priv<16> };
+									   (const void*)k_gmres_priv<32>, (const void*)k_gmres_priv<16>,
+									   (const void*)k_gmres_priv8<32>, (const void*)k_gmres_priv8<16> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
-			if (o.precision == 1 && small && m <= 32 && allow_reg)
+			if (o.precision == 1 && !small && m <= 32 && allow_reg)
+			{
+				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				timed_launch(c, "k_gmres_priv8<float>", by, [&] {
+					if (m <= 16)
+						k_gmres_priv8<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			  o.max_iters, po, lp, o.inner_rtol);
+					else
+						k_gmres_priv8<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			  o.max_iters, po, lp, o.inner_rtol);
+				});
+			}
+			else if (o.precision == 1 && small && m <= 32 && allow_reg)
 			{
 				// the FP32 basis of each thread's rows as a private array
 				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
@@ -3640,7 +3663,7 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 			cands.push_back({ 0, m });
 			if (ilu && P.nU <= lds_max_unknowns) // otherwise path 1 is path 0
 				cands.push_back({ 1, m });
-			if (ilu && o.precision == 1 && P.nU <= 4 * pblock && m <= 32) // otherwise path 3 is path 0
+			if (ilu && o.precision == 1 && P.nU <= lds_max_unknowns && m <= 32) // otherwise path 3 is path 0
 				cands.push_back({ 3, m });
 			cands.push_back({ 2, m });
 		}
