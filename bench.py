@@ -63,6 +63,44 @@ def parse():
     return ap.parse_args()
 
 
+def profile_stats(L, ctx):
+    """(name, launches, ms, algorithmic bytes) of every kernel timed since profiling started."""
+    stats = []
+    name = C.create_string_buffer(128)
+    i = 0
+    while True:
+        launches, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        if L.esl_profile_stat(ctx.handle, i, name, 128, C.byref(launches), C.byref(ms), C.byref(by)) != 0:
+            break
+        stats.append((name.value.decode(), launches.value, ms.value, by.value))
+        i += 1
+    return stats
+
+
+TABLE_KERNELS = ("k_flip_mask", "k_table_fill")
+MASK_PASS_KERNELS = ("k_trim", "k_color", "k_close", "k_height_check")
+
+
+def stage_bandwidth(stats, n, m):
+    """Per-stage device time, algorithmic bytes and GB/s of the table and graph
+    stages.  Graph stage: passes over the per-state flip masks (4 B per state
+    each, plus the labels they read and write); also the compulsory-bytes form
+    4 n * passes + 3 * 4 n (mask per pass; labels, heights, order written once)."""
+    tab = [s for s in stats if s[0] in TABLE_KERNELS]
+    gra = [s for s in stats if s[0] not in TABLE_KERNELS and s[0].startswith("k_")]
+    out = {}
+    for key, ks in (("table", tab), ("graph", gra)):
+        ms = sum(k[2] for k in ks)
+        by = sum(k[3] for k in ks)
+        out[key] = {"kernel_ms": ms, "algorithmic_GB": by / 1e9, "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                    "kernels": {k[0]: {"launches": k[1], "ms": k[2]} for k in ks}}
+    out["table"]["compulsory_GB"] = (4.0 * (n + 1) + 8.0 * m + 4.0 * n) / 1e9  # indptr, rows, cols, masks written
+    passes = sum(k[1] for k in gra if k[0] in MASK_PASS_KERNELS)
+    out["graph"]["mask_passes"] = passes
+    out["graph"]["compulsory_GB"] = (4.0 * n * passes + 12.0 * n) / 1e9
+    return out
+
+
 def traffic_record(kernel, workload):
     """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE passes, scaled by the same counters on a 1 GiB copy), when a
@@ -124,7 +162,9 @@ def main():
 
     x0 = initial_state(model.nodes, fixed, vals).state
 
-    # shared symbolic part: table + graph, built once
+    # shared symbolic part: table + graph, built once (kernel timing on, to report
+    # each stage's algorithmic bandwidth)
+    check(L.esl_profile(ctx.handle, 1), ctx.handle)
     t0 = time.perf_counter()
     table = transition_table(ctx, model, rates[0])
     table.construct_table()
@@ -132,6 +172,9 @@ def main():
     t_graph = time.perf_counter() - t0
     sizes = table.sizes()
     stage_ms = table.stage_ms()
+    stage_stats = profile_stats(L, ctx)
+    check(L.esl_profile(ctx.handle, 0), ctx.handle)
+    stages = stage_bandwidth(stage_stats, n, int(sizes[1]))
 
     def dalloc(nbytes):
         p = C.c_void_p()
@@ -239,16 +282,7 @@ def main():
         reports.append(solve_batch(B, d_rates, d_final, a.precision, tol, graph=a.graph, restart=restart).as_dict())
     sync()
     elapsed = time.perf_counter() - t
-    stats = []
-    i = 0
-    name = C.create_string_buffer(128)
-    while True:
-        launches, ms, by = C.c_int64(), C.c_double(), C.c_double()
-        st = L.esl_profile_stat(ctx.handle, i, name, 128, C.byref(launches), C.byref(ms), C.byref(by))
-        if st != 0:
-            break
-        stats.append((name.value.decode(), launches.value, ms.value, by.value))
-        i += 1
+    stats = profile_stats(L, ctx)
     check(L.esl_profile(ctx.handle, 0), ctx.handle)
     ms_per_step = 1000.0 * elapsed / a.steps
     value = B * a.steps / elapsed
@@ -317,6 +351,7 @@ def main():
         "speedup_vs_plain_fp64": value * plain_s,
         "shared_stages_ms": {"table": float(stage_ms[0]), "graph": float(stage_ms[1]),
                              "wall_table_graph": 1000 * t_graph, "plan": 1000 * t_plan},
+        "stage_bandwidth": stages,
         "plan": {"unknowns": int(info[0]), "terminal_unknowns": int(info[1]), "ambiguous": int(info[2]),
                  "terminals_reached": int(info[3]), "coupling_nnz": int(info[5])},
         "accuracy": {"max_l1_vs_plain_fp64": max(l1), "max_rel_vs_plain_fp64": max(maxrel),

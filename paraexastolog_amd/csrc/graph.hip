@@ -328,6 +328,30 @@ struct pass_flags
 	}
 };
 
+// Runs a fixpoint pass in groups of `group` launches with one host round trip per
+// group: pass p of a group reports a change in its own flag slot; the first pass
+// that changed nothing ends the loop (the passes after it in its group found the
+// fixpoint unchanged).  pass(flag) launches one pass.
+template <typename F>
+void fixpoint(ctx_t& c, int group, int max_passes, const char* what, F&& pass)
+{
+	constexpr int base = 8; // flag slots base .. base + group - 1
+	int32_t* d = c.flags.get() + base;
+	for (int done = 0;; done += group)
+	{
+		ESL_HIP(hipMemsetAsync(d, 0, group * sizeof(int32_t), c.stream));
+		for (int p = 0; p < group; ++p)
+			pass(d + p);
+		ESL_HIP(hipMemcpyAsync(c.h_flags + base, d, group * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		for (int p = 0; p < group; ++p)
+			if (!c.h_flags[base + p])
+				return;
+		if (done > max_passes)
+			throw error(ESL_E_DEVICE, std::string(what) + " did not converge");
+	}
+}
+
 } // namespace
 
 void graph_reorganize(ctx_t& c, graph_t& g)
@@ -347,39 +371,29 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	ESL_HIP(hipMemsetAsync(g.scc.get(), 0xff, n * sizeof(int32_t), c.stream));
 
 	const int max_passes = 1 << 20;
+	const int pass_group = 4; // fixpoint passes per host round trip
 	int64_t prev_left = n + 1;
 	for (int round = 0;; ++round)
 	{
-		for (int p = 0;; ++p)
-		{
-			fl.clear(0);
-			k_trim<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), fl.d);
-			if (!fl.read(0))
-				break;
-			if (p > max_passes)
-				throw error(ESL_E_DEVICE, "trim did not converge");
-		}
+		// algorithmic bytes of one pass over the flip masks: mask (4 B/state) and the
+		// state labels it reads and writes
+		fixpoint(c, pass_group, max_passes, "trim", [&](int32_t* f) {
+			timed_launch(c, "k_trim", 12.0 * n,
+						 [&] { k_trim<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), f); });
+		});
 		k_color_init<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get());
-		for (int p = 0;; ++p)
-		{
-			fl.clear(0);
-			k_color<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), fl.d);
-			if (!fl.read(0))
-				break;
-			if (p > max_passes)
-				throw error(ESL_E_DEVICE, "colouring did not converge");
-		}
-		for (int p = 0;; ++p)
-		{
-			fl.clear(0);
-			k_close<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), inscc.get(), fl.d);
-			if (!fl.read(0))
-				break;
-			if (p > max_passes)
-				throw error(ESL_E_DEVICE, "closure did not converge");
-		}
+		fixpoint(c, pass_group, max_passes, "colouring", [&](int32_t* f) {
+			timed_launch(c, "k_color", 16.0 * n,
+						 [&] { k_color<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), f); });
+		});
+		fixpoint(c, pass_group, max_passes, "closure", [&](int32_t* f) {
+			timed_launch(c, "k_close", 14.0 * n, [&] {
+				k_close<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), inscc.get(), f);
+			});
+		});
 		fl.clear(1);
-		k_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get(), fl.d + 1);
+		timed_launch(c, "k_assign", 13.0 * n,
+					 [&] { k_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get(), fl.d + 1); });
 		ESL_HIP(hipGetLastError());
 		const int32_t left = fl.read(1);
 		if (!left)
@@ -410,9 +424,14 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	while (labelled_total < S)
 	{
 		++cur;
-		k_height_check<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), hroot.get(), blocked.get(), cur);
+		timed_launch(c, "k_height_check", 13.0 * n, [&] {
+			k_height_check<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), hroot.get(), blocked.get(),
+														cur);
+		});
 		fl.clear(2);
-		k_height_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), blocked.get(), cur, fl.d + 2);
+		timed_launch(c, "k_height_assign", 9.0 * n, [&] {
+			k_height_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), blocked.get(), cur, fl.d + 2);
+		});
 		ESL_HIP(hipGetLastError());
 		const int32_t got = fl.read(2);
 		if (got == 0)
