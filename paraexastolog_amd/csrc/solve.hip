@@ -1387,10 +1387,11 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 // coupling structure (shared by all sets, 16-bit indices), the inverse pivots
 // and the vector being preconditioned live in LDS, so a level costs LDS
 // latency plus one workgroup barrier.  Each thread owns the unknowns
-// u = threadIdx.x + i * pblock (i < UPT): the Arnoldi vector w, the basis
-// entries and D' of its own rows stay in registers, and the basis in global
-// memory is only ever touched by the thread that owns the row, so it needs no
-// barrier.  Same method and results as k_gmres_set (ILU(0) only).
+// u = threadIdx.x + i * pblock (i < UPT): the Arnoldi vector w and D' of its
+// own rows stay in registers, and the basis entries of those rows (a private
+// per-thread array, or global memory) are only ever touched by the thread that
+// owns the row, so they need no barrier.  Same method and results as
+// k_gmres_set (ILU(0) only).
 
 struct lds_plan_t
 {
@@ -2180,13 +2181,6 @@ __global__ __launch_bounds__(sblock) void k_copy_live(int64_t nU, const T* __res
 	out[(int64_t)b * nU + u] = (double)a[(int64_t)b * nU + u];
 }
 
-template <typename T>
-__global__ __launch_bounds__(sblock) void k_copy(int64_t count, const T* __restrict__ a, double* __restrict__ b)
-{
-	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	if (i < count)
-		b[i] = (double)a[i];
-}
 
 __global__ __launch_bounds__(sblock) void k_axpy1(int64_t count, const double* __restrict__ a, double* __restrict__ x)
 {
