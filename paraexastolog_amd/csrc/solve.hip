@@ -1022,6 +1022,10 @@ struct nbr_t
 	const uint8_t* nr;	// rate index | mirrored flag
 	const int32_t* lvl; // unknowns grouped by popcount of their state
 	levels_t L;
+	// the same lists with rows in level order, packed for the LDS-resident
+	// kernels (built only for systems that fit them, else null)
+	const uint32_t* hdr; // [nU] first record | lower count << 16 | count << 24 (7 bits)
+	const uint32_t* rec; // [nnz] neighbour position | rate code << 16
 };
 
 // ============================================================================
@@ -1395,8 +1399,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 
 struct lds_plan_t
 {
-	int off_H, off_cs, off_sn, off_g, off_c, off_part, off_t, off_ud, off_lvl, off_ptr, off_nlow, off_nu, off_nc,
-		off_anc, bytes;
+	int off_H, off_g, off_c, off_part, off_t, off_iu, off_hdr, off_rec, bytes;
 };
 
 __host__ __device__ inline int lds_take(int& o, int64_t bytes)
@@ -1406,25 +1409,26 @@ __host__ __device__ inline int lds_take(int& o, int64_t bytes)
 	return at;
 }
 
+// The rotated Hessenberg matrix is upper triangular: column j (rows 0..j)
+// starts at hcol(j) of a packed array.  While column j is formed its
+// subdiagonal entry would sit at hcol(j) + j + 1 = hcol(j + 1), the first entry
+// of the next column, which is written only after that entry has been rotated
+// away.
+__host__ __device__ inline int hcol(int j) { return j * (j + 1) / 2; }
+
 // dynamic LDS carve-up for nU unknowns, nnz couplings, restart m, vectors of psize bytes
 inline lds_plan_t lds_plan(int64_t nU, int64_t nnz, int m, int psize)
 {
 	lds_plan_t p{};
 	int o = 0;
-	p.off_H = lds_take(o, 8 * (int64_t)(m + 1) * m);
-	p.off_cs = lds_take(o, 8 * (int64_t)m);
-	p.off_sn = lds_take(o, 8 * (int64_t)m);
+	p.off_H = lds_take(o, 8 * (int64_t)(hcol(m) + 1));
 	p.off_g = lds_take(o, 8 * (int64_t)(m + 1));
 	p.off_c = lds_take(o, 8 * (int64_t)(m + 1));
 	p.off_part = lds_take(o, 8 * (int64_t)(m + 1) * pwaves);
 	p.off_t = lds_take(o, psize * nU);
-	p.off_ud = lds_take(o, psize * nU);
-	p.off_lvl = lds_take(o, 2 * nU);
-	p.off_ptr = lds_take(o, 2 * (nU + 1));
-	p.off_nlow = lds_take(o, nU);
-	p.off_nu = lds_take(o, 2 * nnz);
-	p.off_nc = lds_take(o, nnz);
-	p.off_anc = lds_take(o, 4 * ((nU + 31) / 32));
+	p.off_iu = lds_take(o, psize * nU);
+	p.off_hdr = lds_take(o, 4 * nU);
+	p.off_rec = lds_take(o, 4 * (nnz + 4));
 	p.bytes = o;
 	return p;
 }
@@ -1432,74 +1436,136 @@ inline lds_plan_t lds_plan(int64_t nU, int64_t nnz, int m, int psize)
 constexpr int lds_max_unknowns = 2048;
 constexpr int lds_static_bytes = 1024; // upper bound of the kernel's static LDS
 
+// The coupling structure of one set in LDS.  Rows are numbered by their
+// position p in the popcount-level order (G.lvl), so a level is a contiguous
+// range of rows and a neighbour is reached without an index translation:
+//   hdr[p] = first record | lower-neighbour count << 16 | neighbour count << 24
+//            | anchor flag << 31
+//   rec[k] = neighbour position | (rate index | mirrored flag) << 16
+// (lower neighbours first, each part in the order of the coupling lists).
+// The anchor flag is set per rate set in LDS, and an anchor's counts are set to
+// zero there: its row of M is the identity and its pivot is 1, so the sweeps
+// and the row product handle it without a branch.  Four zero records follow
+// the last one, so a batch of four records never reads past the array.
+constexpr uint32_t hdr_anchor = 1u << 31;
 struct lds_view_t
 {
-	const uint16_t* lvl;
-	const uint16_t* ptr;
-	const uint8_t* nlow;
-	const uint16_t* nu;
-	const uint8_t* nc;
-	const uint32_t* anc;
+	const uint32_t* hdr;
+	const uint32_t* rec;
 	const double* rates;
-	__device__ bool anchor(int u) const { return (anc[u >> 5] >> (u & 31)) & 1u; }
+	__device__ bool anchor(int p) const { return hdr[p] >> 31; }
 };
+__device__ __forceinline__ int hdr_first(uint32_t h) { return h & 0xffff; }
+__device__ __forceinline__ int hdr_low(uint32_t h) { return (h >> 16) & 0xff; }
+__device__ __forceinline__ int hdr_count(uint32_t h) { return (h >> 24) & 0x7f; }
 
-// in place on LDS: t <- P^{-1} t (ILU(0), inverse pivots iu); ends with a barrier
-template <typename P>
-__device__ void ilu_lds(const levels_t& L, const lds_view_t& v, const P* __restrict__ iu, P* __restrict__ t)
+// s + sign * sum over records k0..k1-1 of rate * y[neighbour], added in record
+// order.  The records of a row are loaded four at a time (records past k1
+// belong to the next rows or to the zero padding, so their positions are in
+// range; their terms are dropped by a select): a row of up to four neighbours
+// costs two dependent LDS round trips instead of two per neighbour.
+template <int SIGN, typename Y>
+__device__ __forceinline__ double rec_sum(const lds_view_t& v, int k0, int k1, const Y* __restrict__ y, double s)
 {
-	for (int l = 0; l < L.n; ++l)
+	for (int k = k0; k < k1; k += 4)
 	{
-		const int a = L.off[l], e = L.off[l + 1];
-		if (a == e)
-			continue;
-		for (int q = a + threadIdx.x; q < e; q += pblock)
+		uint32_t r[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+			r[i] = v.rec[k + i];
+		double rt[4];
+		Y yv[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
 		{
-			const int u = v.lvl[q];
-			if (v.anchor(u))
-				continue;
-			double y = (double)t[u];
-			const int k0 = v.ptr[u], k1 = k0 + v.nlow[u];
-			for (int k = k0; k < k1; ++k)
-			{
-				const int uk = v.nu[k];
-				y += v.rates[v.nc[k] & 63] * (double)iu[uk] * (double)t[uk];
-			}
-			t[u] = (P)y;
+			rt[i] = v.rates[(r[i] >> 16) & 63];
+			yv[i] = y[r[i] & 0xffff];
 		}
-		__syncthreads();
+		// keep the eight loads together (the scheduler would otherwise
+		// interleave them with the products, one round trip per term)
+		__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			const double term = rt[i] * (double)yv[i];
+			s += k + i < k1 ? (SIGN > 0 ? term : -term) : 0.0;
+		}
 	}
-	for (int l = L.n - 1; l >= 0; --l)
+	return s;
+}
+
+// In place on LDS: t <- P^{-1} t for ILU(0) P = (I - L D^{-1}) D (I - D^{-1} U)
+// with inverse pivots iu = D^{-1}, one workgroup barrier per non-empty level.
+// The forward sweep stores iu_p y_p rather than y_p, so that both sweeps read
+// one value per neighbour:
+//   forward  y_p = t_p + sum_lower rate * (iu y)_k,   t_p <- iu_p y_p
+//   backward z_p = t_p + iu_p sum_upper rate * z_k
+// The top non-empty level has no upper neighbours: its forward value is final.
+// The bound of the next level is read while the current one is processed.
+template <typename P>
+__device__ void ilu_lds(int nl, const int* __restrict__ loff, const lds_view_t& v, const P* __restrict__ iu,
+						P* __restrict__ t)
+{
+	const int tid = threadIdx.x;
+	int a = loff[0], e = loff[1];
+	int top = 0; // highest non-empty level
+	for (int l = 0; l < nl; ++l)
 	{
-		const int a = L.off[l], e = L.off[l + 1];
-		if (a == e)
-			continue;
-		for (int q = a + threadIdx.x; q < e; q += pblock)
+		const int e2 = loff[l + 2 <= nl ? l + 2 : nl];
+		if (a < e)
 		{
-			const int u = v.lvl[q];
-			if (v.anchor(u))
-				continue;
-			double z = (double)t[u];
-			const int k0 = v.ptr[u] + v.nlow[u], k1 = v.ptr[u + 1];
-			for (int k = k0; k < k1; ++k)
-				z += v.rates[v.nc[k] & 63] * (double)t[v.nu[k]];
-			t[u] = (P)(z * (double)iu[u]);
+			top = l;
+			for (int q = a + tid; q < e; q += pblock)
+			{
+				const uint32_t h = v.hdr[q];
+				const double tq = (double)t[q], iq = (double)iu[q];
+				const int k0 = hdr_first(h);
+				t[q] = (P)(iq * rec_sum<1>(v, k0, k0 + hdr_low(h), t, tq));
+			}
+			__syncthreads();
 		}
-		__syncthreads();
+		a = e;
+		e = e2;
+	}
+	if (top == 0)
+		return;
+	a = loff[top - 1];
+	e = loff[top];
+	for (int l = top - 1; l >= 0; --l)
+	{
+		const int a2 = loff[l > 0 ? l - 1 : 0];
+		if (a < e)
+		{
+			for (int q = a + tid; q < e; q += pblock)
+			{
+				const uint32_t h = v.hdr[q];
+				const double tq = (double)t[q], iq = (double)iu[q];
+				const int k0 = hdr_first(h);
+				t[q] = (P)(tq + iq * rec_sum<1>(v, k0 + hdr_low(h), k0 + hdr_count(h), t, 0.0));
+			}
+			__syncthreads();
+		}
+		e = a;
+		a = a2;
 	}
 }
 
-// row u of M applied to a vector y held in LDS or global memory (FP64 sum)
+// row p of M applied to a vector y (rows in level order) held in LDS or global
+// memory (FP64 sum)
 template <typename Y>
-__device__ __forceinline__ double row_lds(const lds_view_t& v, int u, double d, const Y* __restrict__ y)
+__device__ __forceinline__ double row_lds(const lds_view_t& v, int p, double d, const Y* __restrict__ y)
 {
-	if (v.anchor(u))
-		return (double)y[u];
-	double acc = d * (double)y[u];
-	const int k1 = v.ptr[u + 1];
-	for (int k = v.ptr[u]; k < k1; ++k)
-		acc -= v.rates[v.nc[k] & 63] * (double)y[v.nu[k]];
-	return acc;
+	const uint32_t h = v.hdr[p];
+	const int k0 = hdr_first(h);
+	return rec_sum<-1>(v, k0, k0 + hdr_count(h), y, d * (double)y[p]);
+}
+
+// value of v in lane i of the wavefront (i uniform)
+__device__ __forceinline__ double lane_value(double v, int i)
+{
+	const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
+	const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+	return __hiloint2double(hi, lo);
 }
 
 constexpr int dot_chunk = 8; // basis vectors per batch of dot products / updates
@@ -1669,7 +1735,11 @@ __device__ __forceinline__ void reg_update(const float (&vb)[MR + 1][UPT], int j
 }
 
 // MR > 0: the FP32 basis of the thread's rows is a private per-thread array
-// (restart length at most MR); MR = 0: the basis lives in global memory
+// (restart length at most MR); MR = 0: the basis lives in global memory.
+// Inside the kernel every vector is indexed by level position (thread tid owns
+// positions tid + i pblock); the FP64 iterate and the right-hand side are
+// gathered into that order at the start (per-set scratch r and udiag) and the
+// result is scattered back at the end.
 template <typename T, int UPT, int MR>
 __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, int n_seg, const double* __restrict__ rhs,
 											   double* __restrict__ zio, const pgm_scratch_t& sc, int m, double tol,
@@ -1679,32 +1749,28 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 	using P = T; // precision of the LDS vectors (preconditioned vector, inverse pivots)
 	extern __shared__ __align__(16) unsigned char smem[];
 	double* s_H = (double*)(smem + lp.off_H);
-	double* s_cs = (double*)(smem + lp.off_cs);
-	double* s_sn = (double*)(smem + lp.off_sn);
 	double* s_g = (double*)(smem + lp.off_g);
 	double* s_c = (double*)(smem + lp.off_c);
 	double* s_part = (double*)(smem + lp.off_part);
 	P* s_t = (P*)(smem + lp.off_t);
-	P* s_iu = (P*)(smem + lp.off_ud);
-	uint16_t* s_lvl = (uint16_t*)(smem + lp.off_lvl);
-	uint16_t* s_ptr = (uint16_t*)(smem + lp.off_ptr);
-	uint8_t* s_nlow = (uint8_t*)(smem + lp.off_nlow);
-	uint16_t* s_nu = (uint16_t*)(smem + lp.off_nu);
-	uint8_t* s_nc = (uint8_t*)(smem + lp.off_nc);
-	uint32_t* s_anc = (uint32_t*)(smem + lp.off_anc);
+	P* s_iu = (P*)(smem + lp.off_iu);
+	uint32_t* s_hdr = (uint32_t*)(smem + lp.off_hdr);
+	uint32_t* s_rec = (uint32_t*)(smem + lp.off_rec);
 	__shared__ double s_red[pwaves];
+	__shared__ double s_nrm[pwaves];
 	__shared__ double s_scalar[4];
 	__shared__ double s_rates[64];
+	__shared__ int s_loff[34];
 
 	const int b = blockIdx.x;
-	const int tid = threadIdx.x;
+	const int tid = threadIdx.x, lane = tid & (wave - 1), wv = tid / wave;
 	const int nU = (int)S.nU;
+	const int nl = G.L.n;
 	const int64_t off = (int64_t)b * nU;
-	const double* __restrict__ rb = rhs + off;
-	double* __restrict__ x = zio + off; // z = D' x on entry and exit, x inside
-	const double* __restrict__ ib = S.invd + off;
+	double* __restrict__ xp = sc.r + off;	   // FP64 iterate x = D'^{-1} z
+	double* __restrict__ rbp = sc.udiag + off; // right-hand side
+	double* __restrict__ corr = sc.t + off;	   // last correction (undone when an FP32 cycle made things worse)
 	T* __restrict__ V = (T*)sc.V + (int64_t)b * (m + 1) * nU;
-	const int mh = m + 1; // column stride of s_H
 	static_assert(MR == 0 || sizeof(T) == 4, "the private basis is FP32");
 	float vb[MR > 0 ? MR + 1 : 1][UPT];
 #pragma unroll
@@ -1713,96 +1779,88 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		for (int i = 0; i < UPT; ++i)
 			vb[ii][i] = 0.0f;
 
-	// ---- coupling structure, rates and anchor flags into LDS
+	// ---- coupling structure, rates, level bounds into LDS
 	const int nnz = G.ptr[nU];
-	for (int q = tid; q < nU; q += pblock)
-	{
-		s_lvl[q] = (uint16_t)G.lvl[q];
-		s_ptr[q] = (uint16_t)G.ptr[q];
-		s_nlow[q] = (uint8_t)(G.mid[q] - G.ptr[q]);
-	}
-	if (tid == 0)
-		s_ptr[nU] = (uint16_t)nnz;
-	for (int k = tid; k < nnz; k += pblock)
-	{
-		s_nu[k] = (uint16_t)G.nu[k];
-		s_nc[k] = G.nr[k];
-	}
+	for (int k = tid; k < nnz + 4; k += pblock)
+		s_rec[k] = k < nnz ? G.rec[k] : 0u;
 	for (int i = tid; i < 2 * S.N; i += pblock)
 		s_rates[i] = S.rates[(int64_t)b * S.rate_stride + i];
-	for (int w = tid; w < (nU + 31) / 32; w += pblock)
-		s_anc[w] = 0u;
-	__syncthreads();
-	// one row per thread (independent loads), bits set with LDS atomics
-	for (int u = tid; u < nU; u += pblock)
-		if (is_anchor(S, n_seg, b, u))
-			atomicOr(&s_anc[u >> 5], 1u << (u & 31));
-	double dg[UPT]; // D' of the own rows
+	for (int i = tid; i <= nl; i += pblock)
+		s_loff[i] = G.L.off[i];
+	// ---- own rows: headers with the anchor flag, D', x and b in level order
+	double dg[UPT];
+	double bb = 0.0;
 #pragma unroll
 	for (int i = 0; i < UPT; ++i)
 	{
-		const int u = tid + i * pblock;
+		const int p = tid + i * pblock;
 		dg[i] = 1.0;
-		if (u < nU)
+		if (p < nU)
 		{
-			dg[i] = 1.0 / ib[u];
-			x[u] *= ib[u];
+			const int u = G.lvl[p];
+			const double ibu = S.invd[off + u];
+			const double r = rhs[off + u];
+			dg[i] = 1.0 / ibu;
+			xp[p] = zio[off + u] * ibu;
+			rbp[p] = r;
+			bb += r * r;
+			s_iu[p] = (P)dg[i]; // D' until the pivots replace it
+			const uint32_t h = G.hdr[p];
+			s_hdr[p] = is_anchor(S, n_seg, b, u) ? (h & 0xffffu) | hdr_anchor : h;
 		}
 	}
 	__threadfence_block();
-	__syncthreads();
-	const lds_view_t lv{ s_lvl, s_ptr, s_nlow, s_nu, s_nc, s_anc, s_rates };
+	bb = pg_sum(bb, s_red);
+	const double bn = bb > 0.0 ? sqrt(bb) : 1.0;
+	const lds_view_t lv{ s_hdr, s_rec, s_rates };
 
 	// ---- inverse ILU(0) pivots, level by level
-	for (int l = 0; l < G.L.n; ++l)
+	for (int l = 0; l < nl; ++l)
 	{
-		const int a = G.L.off[l], e = G.L.off[l + 1];
+		const int a = s_loff[l], e = s_loff[l + 1];
 		if (a == e)
 			continue;
 		for (int q = a + tid; q < e; q += pblock)
 		{
-			const int u = s_lvl[q];
-			double d = 1.0 / ib[u];
-			if (!lv.anchor(u))
+			double d = (double)s_iu[q];
+			const uint32_t h = s_hdr[q];
+			if (!(h & hdr_anchor))
 			{
-				const int k0 = s_ptr[u], k1 = k0 + s_nlow[u];
+				const int k0 = hdr_first(h), k1 = k0 + hdr_low(h);
 				for (int k = k0; k < k1; ++k)
 				{
-					const int code = s_nc[k];
-					const int uk = s_nu[k];
-					if ((code & nbr_rev_flag) && !lv.anchor(uk))
-						d -= s_rates[code & 63] * s_rates[(code & 63) ^ 1] * (double)s_iu[uk];
+					const uint32_t r = s_rec[k];
+					const int code = (r >> 16) & 0xff;
+					const int pk = r & 0xffff;
+					if ((code & nbr_rev_flag) && !lv.anchor(pk))
+						d -= s_rates[code & 63] * s_rates[(code & 63) ^ 1] * (double)s_iu[pk];
 				}
 			}
-			s_iu[u] = (P)(1.0 / d);
+			s_iu[q] = (P)(1.0 / d);
 		}
 		__syncthreads();
 	}
 
 	int iters = 0, cycles = 0, stalled = 0;
 	double rel = 0.0, rel_prev = 0.0;
-	double* __restrict__ corr = sc.t + off; // last correction (undone when an FP32 cycle made things worse)
 	for (;;)
 	{
 		// FP64 residual of the own rows
 		double ro[UPT];
-		double rr = 0.0, bb = 0.0;
+		double rr = 0.0;
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
 		{
-			const int u = tid + i * pblock;
+			const int p = tid + i * pblock;
 			ro[i] = 0.0;
-			if (u < nU)
+			if (p < nU)
 			{
-				ro[i] = rb[u] - row_lds<double>(lv, u, dg[i], x);
+				ro[i] = rbp[p] - row_lds<double>(lv, p, dg[i], xp);
 				rr += ro[i] * ro[i];
-				bb += rb[u] * rb[u];
 			}
 		}
 		rr = pg_sum(rr, s_red);
-		bb = pg_sum(bb, s_red);
 		const double beta = sqrt(rr);
-		const double bn = bb > 0.0 ? sqrt(bb) : 1.0;
 		rel = beta / bn;
 		if (rel <= tol || iters >= max_iters)
 			break;
@@ -1815,9 +1873,9 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 #pragma unroll
 				for (int i = 0; i < UPT; ++i)
 				{
-					const int u = tid + i * pblock;
-					if (u < nU)
-						x[u] -= corr[u];
+					const int p = tid + i * pblock;
+					if (p < nU)
+						xp[p] -= corr[p];
 				}
 				rel = rel_prev;
 			}
@@ -1834,19 +1892,16 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
 		{
-			const int u = tid + i * pblock;
+			const int p = tid + i * pblock;
 			vj[i] = (T)(ro[i] * inv_beta);
 			if constexpr (MR > 0)
 				vb[0][i] = (float)vj[i];
-			else if (u < nU)
-				V[u] = vj[i];
+			else if (p < nU)
+				V[p] = vj[i];
 		}
-		if (tid == 0)
-		{
-			for (int i = 0; i <= m; ++i)
-				s_g[i] = 0.0;
-			s_g[0] = beta;
-		}
+		// wavefront 0 applies the Givens rotations: lane i keeps rotation i of
+		// this cycle, g_run is the last entry of the rotated right-hand side
+		double g_run = beta, cs_l = 1.0, sn_l = 0.0;
 
 		int k = 0;
 		for (int j = 0; j < m; ++j)
@@ -1855,22 +1910,23 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 #pragma unroll
 			for (int i = 0; i < UPT; ++i)
 			{
-				const int u = tid + i * pblock;
-				if (u < nU)
-					s_t[u] = (P)vj[i];
+				const int p = tid + i * pblock;
+				if (p < nU)
+					s_t[p] = (P)vj[i];
 			}
 			__syncthreads();
-			ilu_lds<P>(G.L, lv, s_iu, s_t);
+			ilu_lds<P>(nl, s_loff, lv, s_iu, s_t);
 			// w = M t on the own rows
 			double w[UPT];
 #pragma unroll
 			for (int i = 0; i < UPT; ++i)
 			{
-				const int u = tid + i * pblock;
-				w[i] = u < nU ? row_lds<P>(lv, u, dg[i], s_t) : 0.0;
+				const int p = tid + i * pblock;
+				w[i] = p < nU ? row_lds<P>(lv, p, dg[i], s_t) : 0.0;
 			}
-			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
-			double w2 = 0.0;
+			// classical Gram-Schmidt, twice; lane i <= j of wavefront 0
+			// accumulates H(i, j) in hc
+			double hc = 0.0;
 			for (int pass = 0; pass < 2; ++pass)
 			{
 				if constexpr (MR > 0)
@@ -1884,41 +1940,57 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 					for (int q = 0; q < pwaves; ++q)
 						tt += s_part[tid * pwaves + q];
 					s_c[tid] = tt;
-					s_H[j * mh + tid] = (pass == 0 ? 0.0 : s_H[j * mh + tid]) + tt;
+					hc += tt;
 				}
 				__syncthreads();
-				w2 = 0.0;
 				if constexpr (MR > 0)
 					reg_update<UPT, MR>(vb, j, s_c, w);
 				else
 					wave_update<T, UPT>(V, nU, j, s_c, w);
-#pragma unroll
-				for (int i = 0; i < UPT; ++i)
-					w2 += w[i] * w[i];
-				w2 = pg_sum(w2, s_red);
 			}
-			if (tid == 0)
+			double w2 = 0.0;
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+				w2 += w[i] * w[i];
+			w2 = wave_sum(w2);
+			if (lane == 0)
+				s_nrm[wv] = w2;
+			__syncthreads();
+			if (wv == 0)
 			{
-				double* col = s_H + j * mh;
-				const double hn = sqrt(w2);
-				col[j + 1] = hn;
+				double nn = 0.0;
+				for (int q = 0; q < pwaves; ++q)
+					nn += s_nrm[q];
+				const double hn = sqrt(nn);
+				// rotations 0..j-1 applied to column j, the running entry in a register
+				double a = lane_value(hc, 0);
+				double hv = hc;
 				for (int i = 0; i < j; ++i)
 				{
-					const double a = col[i], c = col[i + 1];
-					col[i] = s_cs[i] * a + s_sn[i] * c;
-					col[i + 1] = -s_sn[i] * a + s_cs[i] * c;
+					const double c = lane_value(hc, i + 1);
+					const double ci = lane_value(cs_l, i), si = lane_value(sn_l, i);
+					const double ni = ci * a + si * c;
+					if (lane == i)
+						hv = ni;
+					a = -si * a + ci * c;
 				}
-				const double a = col[j], c = col[j + 1];
-				const double rn = hypot(a, c);
-				const double cj = rn > 0.0 ? a / rn : 1.0, sj = rn > 0.0 ? c / rn : 0.0;
-				s_cs[j] = cj;
-				s_sn[j] = sj;
-				col[j] = rn;
-				col[j + 1] = 0.0;
-				s_g[j + 1] = -sj * s_g[j];
-				s_g[j] = cj * s_g[j];
-				s_scalar[1] = fabs(s_g[j + 1]) / bn;
-				s_scalar[2] = (hn > 0.0 && rn > 0.0) ? 1.0 / hn : 0.0;
+				const double rn = hypot(a, hn);
+				const double cj = rn > 0.0 ? a / rn : 1.0, sj = rn > 0.0 ? hn / rn : 0.0;
+				if (lane == j)
+				{
+					hv = rn;
+					cs_l = cj;
+					sn_l = sj;
+				}
+				if (lane <= j)
+					s_H[hcol(j) + lane] = hv;
+				if (lane == 0)
+				{
+					s_g[j] = cj * g_run;
+					s_scalar[1] = fabs(sj * g_run) / bn;
+					s_scalar[2] = (hn > 0.0 && rn > 0.0) ? 1.0 / hn : 0.0;
+				}
+				g_run = -sj * g_run;
 			}
 			__syncthreads();
 			const double scale = s_scalar[2];
@@ -1926,11 +1998,11 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 #pragma unroll
 			for (int i = 0; i < UPT; ++i)
 			{
-				const int u = tid + i * pblock;
+				const int p = tid + i * pblock;
 				vj[i] = (T)(w[i] * scale);
 				if constexpr (MR == 0)
-					if (u < nU)
-						Vn[u] = vj[i];
+					if (p < nU)
+						Vn[p] = vj[i];
 			}
 			if constexpr (MR > 0)
 			{
@@ -1946,22 +2018,22 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 			if (s_scalar[1] <= inner_target || scale == 0.0 || iters >= max_iters)
 				break;
 		}
-		// y = H^-1 g;  x += P^{-1} (sum_i y_i V_i)
+		// y = R^{-1} g;  x += P^{-1} (sum_i y_i V_i)
 		if (tid == 0)
 			for (int i = k - 1; i >= 0; --i)
 			{
 				double tt = s_g[i];
 				for (int l = i + 1; l < k; ++l)
-					tt -= s_H[l * mh + i] * s_c[l];
-				const double d = s_H[i * mh + i];
+					tt -= s_H[hcol(l) + i] * s_c[l];
+				const double d = s_H[hcol(i) + i];
 				s_c[i] = d != 0.0 ? tt / d : 0.0;
 			}
 		__syncthreads();
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
 		{
-			const int u = tid + i * pblock;
-			if (u < nU)
+			const int p = tid + i * pblock;
+			if (p < nU)
 			{
 				double acc = 0.0;
 				if constexpr (MR > 0)
@@ -1976,32 +2048,36 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 				}
 				else
 					for (int ii = 0; ii < k; ++ii)
-						acc += s_c[ii] * (double)V[(int64_t)ii * nU + u];
-				s_t[u] = (P)acc;
+						acc += s_c[ii] * (double)V[(int64_t)ii * nU + p];
+				s_t[p] = (P)acc;
 			}
 		}
 		__syncthreads();
-		ilu_lds<P>(G.L, lv, s_iu, s_t);
+		ilu_lds<P>(nl, s_loff, lv, s_iu, s_t);
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
 		{
-			const int u = tid + i * pblock;
-			if (u < nU)
+			const int p = tid + i * pblock;
+			if (p < nU)
 			{
-				const double cu = (double)s_t[u];
-				corr[u] = cu;
-				x[u] += cu;
+				const double cu = (double)s_t[p];
+				corr[p] = cu;
+				xp[p] += cu;
 			}
 		}
 		__threadfence_block();
 		__syncthreads();
 	}
+	// z = D' x, back in the unknown order of the caller
 #pragma unroll
 	for (int i = 0; i < UPT; ++i)
 	{
-		const int u = tid + i * pblock;
-		if (u < nU)
-			x[u] /= ib[u];
+		const int p = tid + i * pblock;
+		if (p < nU)
+		{
+			const int u = G.lvl[p];
+			zio[off + u] = xp[p] / S.invd[off + u];
+		}
 	}
 	if (tid == 0)
 	{
@@ -2011,7 +2087,6 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		out.stalled[b] = stalled;
 	}
 }
-
 
 // The variants differ in where the FP32 Krylov basis of a set lives: in global
 // memory (MR = 0), or as a per-thread array of MR + 1 vectors of its own rows
@@ -2362,9 +2437,15 @@ struct nbr_store_t
 	bool ready = false;
 	dbuf<int32_t> ptr, mid, nu, lvl;
 	dbuf<uint8_t> nr;
+	dbuf<uint32_t> hdr, rec;
+	bool packed = false; // hdr / rec built
 	levels_t L{};
 	int64_t nnz = 0;
-	nbr_t view() const { return nbr_t{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L }; }
+	nbr_t view() const
+	{
+		return nbr_t{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L,
+					  packed ? hdr.get() : nullptr, packed ? rec.get() : nullptr };
+	}
 };
 
 void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
@@ -2402,6 +2483,37 @@ void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
 	for (int64_t u = 0; u < nU; ++u)
 		h_lvl[fill[h_key[u]]++] = (int32_t)u;
 	write_dev(c, st.lvl.get(), h_lvl);
+	// level-ordered packed records for the LDS-resident kernels: positions and
+	// record offsets must fit 16 bits, counts 7 bits (a row has at most N
+	// neighbours)
+	st.packed = false;
+	if (nU <= lds_max_unknowns && nnz < 65536 && S.N < 128)
+	{
+		std::vector<int32_t> h_ptr, h_mid, h_nu;
+		std::vector<uint8_t> h_nr;
+		read_host(c, h_ptr, st.ptr.get(), (size_t)nU + 1);
+		read_host(c, h_mid, st.mid.get(), (size_t)nU);
+		read_host(c, h_nu, st.nu.get(), (size_t)nnz);
+		read_host(c, h_nr, st.nr.get(), (size_t)nnz);
+		std::vector<int32_t> pos(nU);
+		for (int64_t p = 0; p < nU; ++p)
+			pos[h_lvl[p]] = (int32_t)p;
+		std::vector<uint32_t> h_hdr(nU), h_rec(std::max<int64_t>(nnz, 1));
+		uint32_t at = 0;
+		for (int64_t p = 0; p < nU; ++p)
+		{
+			const int32_t u = h_lvl[p];
+			const uint32_t n_low = (uint32_t)(h_mid[u] - h_ptr[u]), cnt = (uint32_t)(h_ptr[u + 1] - h_ptr[u]);
+			h_hdr[p] = at | n_low << 16 | cnt << 24;
+			for (int32_t k = h_ptr[u]; k < h_ptr[u + 1]; ++k)
+				h_rec[at++] = (uint32_t)pos[h_nu[k]] | (uint32_t)h_nr[k] << 16;
+		}
+		st.hdr.ensure(nU);
+		st.rec.ensure(h_rec.size());
+		write_dev(c, st.hdr.get(), h_hdr);
+		write_dev(c, st.rec.get(), h_rec);
+		st.packed = true;
+	}
 	ESL_HIP(hipStreamSynchronize(c.stream));
 	st.ready = true;
 }
@@ -2698,7 +2810,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const nbr_t G = nb.view();
 	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get(), sc.stl.get() };
 	const double by = gmres_set_bytes(S, nb.nnz);
-	if (allow_lds && ilu && S.nU <= lds_max_unknowns && nb.nnz < 65536)
+	if (allow_lds && ilu && nb.packed)
 	{
 		// LDS-resident kernel when its structure, pivots and vector fit
 		const int psize = o.precision == 1 ? 4 : 8;
