@@ -2121,6 +2121,127 @@ __global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(4))) voi
 	gmres_lds_body<float, 4, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
 }
 
+// Anchor choice from the level-ordered packed records of the LDS-resident
+// solver: the damped Jacobi sweeps of k_anchor_sweep (p' = p/2 + inflow/(2D))
+// over every unknown of the set in LDS, then per segment the unknown of
+// largest mass, ties to the smallest index (k_anchor_pick).  Masses are kept
+// in FP32 (they only rank the states), sums in FP64; 1/D of the own rows stays
+// in registers and the records of a row are read four at a time.  Rows of
+// other segments and ambiguous rows are swept along: their lists hold rows of
+// their own part only.  Dynamic LDS: headers, records (+ 4 zero), two FP32
+// mass vectors.
+template <int UPT>
+__global__ __launch_bounds__(pblock) void k_anchor_lvl(sys_t S, nbr_t G, const int32_t* __restrict__ seg_off, int n_seg,
+													   int sweeps, int32_t* __restrict__ anchor)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	const int nU = (int)S.nU;
+	const int nnz = G.ptr[nU];
+	uint32_t* s_hdr = (uint32_t*)smem;
+	uint32_t* s_rec = s_hdr + nU;
+	float* pa = (float*)(s_rec + nnz + 4);
+	float* pb = pa + nU;
+	__shared__ double s_rates[64];
+	__shared__ double s_v[pwaves];
+	__shared__ int32_t s_i[pwaves];
+	const int b = blockIdx.x, tid = threadIdx.x, lane = tid & (wave - 1), wv = tid / wave;
+	for (int k = tid; k < nnz + 4; k += pblock)
+		s_rec[k] = k < nnz ? G.rec[k] : 0u;
+	for (int i = tid; i < 64; i += pblock)
+		s_rates[i] = i < 2 * S.N ? S.rates[(int64_t)b * S.rate_stride + i] : 0.0;
+	__syncthreads();
+	double invd[UPT];
+	int uu[UPT];
+#pragma unroll
+	for (int i = 0; i < UPT; ++i)
+	{
+		const int p = tid + i * pblock;
+		invd[i] = 0.0;
+		uu[i] = -1;
+		if (p < nU)
+		{
+			const int u = G.lvl[p];
+			const uint32_t st = (uint32_t)S.ulist[u];
+			uu[i] = u;
+			s_hdr[p] = G.hdr[p];
+			invd[i] = 1.0 / out_rate_s(st, S.mask[st], s_rates);
+			pa[p] = 1.0f;
+		}
+	}
+	__syncthreads();
+	const lds_view_t lv{ s_hdr, s_rec, s_rates };
+	for (int it = 0; it < sweeps; ++it)
+	{
+		double in[UPT];
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int p = tid + i * pblock;
+			in[i] = 0.0;
+			if (p < nU)
+			{
+				const uint32_t h = lv.hdr[p];
+				in[i] = rec_sum<1>(lv, hdr_first(h), hdr_first(h) + hdr_count(h), pa, 0.0);
+			}
+		}
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int p = tid + i * pblock;
+			if (p < nU)
+				pb[p] = (float)(0.5 * (double)pa[p] + 0.5 * in[i] * invd[i]);
+		}
+		__syncthreads();
+		float* t = pa;
+		pa = pb;
+		pb = t;
+	}
+	for (int k = 0; k < n_seg; ++k)
+	{
+		const int32_t a = seg_off[k], e = seg_off[k + 1];
+		double best = -1.0;
+		int32_t bi = a;
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+			if (uu[i] >= a && uu[i] < e)
+			{
+				const double v = (double)pa[tid + i * pblock];
+				if (v > best || (v == best && uu[i] < bi))
+				{
+					best = v;
+					bi = uu[i];
+				}
+			}
+		for (int o = wave / 2; o > 0; o >>= 1)
+		{
+			const double v2 = __shfl_xor(best, o, wave);
+			const int32_t i2 = __shfl_xor(bi, o, wave);
+			if (v2 > best || (v2 == best && i2 < bi))
+			{
+				best = v2;
+				bi = i2;
+			}
+		}
+		if (lane == 0)
+		{
+			s_v[wv] = best;
+			s_i[wv] = bi;
+		}
+		__syncthreads();
+		if (tid == 0)
+		{
+			for (int w = 1; w < pwaves; ++w)
+				if (s_v[w] > best || (s_v[w] == best && s_i[w] < bi))
+				{
+					best = s_v[w];
+					bi = s_i[w];
+				}
+			anchor[(int64_t)b * n_seg + k] = bi;
+		}
+		__syncthreads();
+	}
+}
+
 // ============================================================================
 // device-wide GMRES kernels (nbr_t coupling lists, iterate x unpreconditioned)
 
@@ -2953,6 +3074,7 @@ struct plan_t
 	per_set_scratch_t scr;
 	int anchor_sweeps = 32;
 	size_t anchor_lds = 0; // dynamic LDS of k_anchor_lds, 0 when the terminal unknowns do not fit
+	size_t anchor_lvl = 0; // dynamic LDS of k_anchor_lvl, 0 without packed records
 	int64_t epoch = 0; // bumped whenever a per-run buffer is reallocated
 	// schedule chosen by plan_autotune, used when a solve asks for restart 0
 	struct tune_t
@@ -3221,6 +3343,7 @@ void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 	// ---- coupling structure of the per-set solver
 	P.nbr.ready = false;
 	P.anchor_lds = 0;
+	P.anchor_lvl = 0;
 	if (P.nU)
 	{
 		sys_t S = P.sys(1, nullptr);
@@ -3239,6 +3362,18 @@ void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 				if (bytes > 48 * 1024)
 					ESL_HIP(hipFuncSetAttribute((const void*)k_anchor_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
 												(int)bytes));
+			}
+		}
+		// with the solver's packed records: headers, records and two FP32 mass vectors
+		if (P.nbr.packed && P.n_seg > 0)
+		{
+			const size_t bytes = 12 * (size_t)P.nU + 4 * ((size_t)P.nbr.nnz + 4) + 16;
+			if (bytes + 1024 <= (size_t)std::min(c.lds_per_block, 160 * 1024))
+			{
+				P.anchor_lvl = bytes;
+				if (bytes > 48 * 1024)
+					for (const void* f : { (const void*)k_anchor_lvl<4>, (const void*)k_anchor_lvl<8> })
+						ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 			}
 		}
 	}
@@ -3281,6 +3416,18 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 		if (warm)
 			// the prediction picks the anchors (no Jacobi sweeps)
 			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, P.d_tpos.get(), o.warm_pi, P.p0.get());
+		else if (P.anchor_lvl)
+		{
+			// sweeps and pick in one launch on the solver's level-ordered records
+			timed_launch(c, "k_anchor_lvl", 8.0 * B * nUt, [&] {
+				if (nU <= 4 * pblock)
+					k_anchor_lvl<4><<<B, pblock, P.anchor_lvl, c.stream>>>(S, P.nbr.view(), P.d_seg_off.get(), n_seg,
+																		   P.anchor_sweeps, P.anchor.get());
+				else
+					k_anchor_lvl<8><<<B, pblock, P.anchor_lvl, c.stream>>>(S, P.nbr.view(), P.d_seg_off.get(), n_seg,
+																		   P.anchor_sweeps, P.anchor.get());
+			});
+		}
 		else if (P.anchor_lds)
 		{
 			// sweeps and pick in one launch, the terminal unknowns in LDS
@@ -3300,7 +3447,7 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 				std::swap(P.p0, P.p1);
 			}
 		}
-		if (warm || !P.anchor_lds)
+		if (warm || (!P.anchor_lds && !P.anchor_lvl))
 			k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.p0.get(), n_seg,
 																	 P.anchor.get());
 	}
