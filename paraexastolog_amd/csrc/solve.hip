@@ -2941,10 +2941,11 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			const bool small = S.nU <= 4 * pblock;
 			if (lp.bytes > 48 * 1024)
 			{
-				const void* fns[8] = { (const void*)k_gmres_lds<float, 4>, (const void*)k_gmres_lds<float, 8>,
-									   (const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
-									   (const void*)k_gmres_priv<32>, (const void*)k_gmres_priv<16>,
-									   (const void*)k_gmres_priv8<32>, (const void*)k_gmres_priv8<16> };
+				const void* fns[10] = { (const void*)k_gmres_lds<float, 4>,  (const void*)k_gmres_lds<float, 8>,
+										(const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
+										(const void*)k_gmres_priv<32>,		 (const void*)k_gmres_priv<24>,
+										(const void*)k_gmres_priv<16>,		 (const void*)k_gmres_priv8<32>,
+										(const void*)k_gmres_priv8<24>,		 (const void*)k_gmres_priv8<16> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
@@ -2954,6 +2955,9 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 				timed_launch(c, "k_gmres_priv8<float>", by, [&] {
 					if (m <= 16)
 						k_gmres_priv8<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			  o.max_iters, po, lp, o.inner_rtol);
+					else if (m <= 24)
+						k_gmres_priv8<24><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
 																			  o.max_iters, po, lp, o.inner_rtol);
 					else
 						k_gmres_priv8<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -2967,6 +2971,9 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 				timed_launch(c, "k_gmres_priv<float>", by, [&] {
 					if (m <= 16)
 						k_gmres_priv<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
+																			 o.max_iters, po, lp, o.inner_rtol);
+					else if (m <= 24)
+						k_gmres_priv<24><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
 																			 o.max_iters, po, lp, o.inner_rtol);
 					else
 						k_gmres_priv<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -3911,7 +3918,7 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 	if (P.nU == 0)
 		cands.push_back({ 0, 32 });
 	else
-		for (int m : { 16, 32, 64 })
+		for (int m : { 16, 24, 32, 64 })
 		{
 			cands.push_back({ 0, m });
 			if (ilu && P.nU <= lds_max_unknowns) // otherwise path 1 is path 0
