@@ -652,6 +652,15 @@ __global__ __launch_bounds__(sblock) void k_zero(int64_t count, T* __restrict__ 
 		p[i] = (T)0;
 }
 
+// the dense final vectors: one 16-byte store per thread (6.9 TB/s on MI355X
+// against 6.4 with 8-byte stores; grid-stride and non-temporal forms were slower)
+__global__ __launch_bounds__(sblock) void k_zero2(int64_t count2, double2* __restrict__ p)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count2)
+		p[i] = make_double2(0.0, 0.0);
+}
+
 // ============================================================================
 // anchors: damped Jacobi sweeps p <- p/2 + (sum_t rate(t->s) p_t / D_s)/2 on
 // every terminal segment, then the argmax of each segment per set.
@@ -3496,7 +3505,11 @@ void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double
 														  P.segsum.get());
 	// algorithmic bytes: the dense final vectors written once
 	timed_launch(c, "k_zero<final>", 8.0 * B * n, [&] {
-		k_zero<double><<<ceil_div((int64_t)B * n, sblock), sblock, 0, c.stream>>>((int64_t)B * n, d_final);
+		const int64_t count = (int64_t)B * n;
+		if (count % 2 == 0 && ((uintptr_t)d_final & 15) == 0)
+			k_zero2<<<ceil_div(count / 2, sblock), sblock, 0, c.stream>>>(count / 2, (double2*)d_final);
+		else
+			k_zero<double><<<ceil_div(count, sblock), sblock, 0, c.stream>>>(count, d_final);
 	});
 	if (n_t && Tr)
 		k_final<<<dim3(ceil_div(n_t, sblock), B), sblock, 0, c.stream>>>(
