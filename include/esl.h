@@ -231,7 +231,7 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
 
 /* Autotuner: runs this batch under each candidate schedule -- per-set solve with
  * the system in LDS, per-set from global memory, device-wide -- and restart length
- * 16 / 32 / 64, keeps the fastest that meets opts->tol in the plan, and returns it
+ * 16 / 24 / 32 / 64, keeps the fastest that meets opts->tol in the plan, and returns it
  * (choice[0] = schedule 0/1/2, choice[1] = restart; *best_ms = its time per batch).
  * d_final receives the batch's results.  Later esl_plan_solve / esl_plan_solve_many
  * calls with opts->restart == 0 use the stored choice (esl_plan_query info[7] =

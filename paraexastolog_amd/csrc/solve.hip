@@ -1933,10 +1933,14 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 				const int p = tid + i * pblock;
 				w[i] = p < nU ? row_lds<P>(lv, p, dg[i], s_t) : 0.0;
 			}
-			// classical Gram-Schmidt, twice; lane i <= j of wavefront 0
-			// accumulates H(i, j) in hc
+			// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles
+			// (an FP32 basis is orthogonal to FP32 accuracy either way, and
+			// the FP64 residual that ends each cycle corrects what the cycle
+			// leaves: the second pass cost 18 % of the kernel and saved no
+			// iterations); lane i <= j of wavefront 0 accumulates H(i, j) in hc
 			double hc = 0.0;
-			for (int pass = 0; pass < 2; ++pass)
+			constexpr int passes = sizeof(T) == 4 ? 1 : 2;
+			for (int pass = 0; pass < passes; ++pass)
 			{
 				if constexpr (MR > 0)
 					reg_dots<UPT, MR>(vb, j, w, s_part);
