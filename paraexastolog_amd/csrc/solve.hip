@@ -1280,8 +1280,10 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 				W[u] = (T)row_apply<double>(S, G, n_seg, b, u, t, s_rates);
 			__threadfence_block();
 			__syncthreads();
-			// classical Gram-Schmidt, twice; coefficients accumulate in s_H[:, j]
-			for (int pass = 0; pass < 2; ++pass)
+			// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles (as in
+			// the LDS-resident kernels); coefficients accumulate in s_H[:, j]
+			constexpr int passes = sizeof(T) == 4 ? 1 : 2;
+			for (int pass = 0; pass < passes; ++pass)
 			{
 				for (int i = 0; i <= j; ++i)
 				{
@@ -1314,7 +1316,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 					w2 += (double)wt * (double)wt;
 				}
 				w2 = pg_sum(w2, s_red);
-				if (pass == 1 && threadIdx.x == 0)
+				if (pass == passes - 1 && threadIdx.x == 0)
 					s_scalar[0] = sqrt(w2);
 				__threadfence_block();
 				__syncthreads();
@@ -2708,6 +2710,8 @@ struct gmres_t
 						 hn.get(),	 beta.get(), bnorm.get(), res.get(), live.get(), act.get(),
 						 kb.get(),	 res0.get() };
 		h_res.resize(B);
+		if (sizeof(T) == 4) // one Gram-Schmidt pass: the second-pass coefficients stay zero
+			k_zero<double><<<ceil_div(B * (m + 1), sblock), sblock, 0, c.stream>>>(B * (m + 1), c2.get());
 	}
 
 	double max_res()
@@ -2812,7 +2816,8 @@ struct gmres_t
 							 [&] {
 								 k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get(), live.get());
 							 });
-				// classical Gram-Schmidt, twice
+				// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles (the
+				// second-pass coefficients c2 then stay zero)
 				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
 				timed_launch(c, "k_dots", dbytes, [&] {
 					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get(),
@@ -2823,11 +2828,15 @@ struct gmres_t
 					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
 																	 m + 1, part.get(), live.get());
 				});
-				k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get(),
-															 live.get());
-				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1, 0);
-				k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c2.get(),
-																 m + 1, part.get(), live.get());
+				if (sizeof(T) != 4)
+				{
+					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride,
+																 part.get(), live.get());
+					k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1,
+																		  0);
+					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride,
+																	 c2.get(), m + 1, part.get(), live.get());
+				}
 				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
 				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st, 0.5 * tol, sizeof(T) == 4 ? inner_rtol : 0.0);
 				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride,
