@@ -1505,44 +1505,83 @@ __device__ __forceinline__ double rec_sum(const lds_view_t& v, int k0, int k1, c
 	return s;
 }
 
-// In place on LDS: t <- P^{-1} t for ILU(0) P = (I - L D^{-1}) D (I - D^{-1} U)
-// with inverse pivots iu = D^{-1}, one workgroup barrier per non-empty level.
-// The forward sweep stores iu_p y_p rather than y_p, so that both sweeps read
-// one value per neighbour:
-//   forward  y_p = t_p + sum_lower rate * (iu y)_k,   t_p <- iu_p y_p
-//   backward z_p = t_p + iu_p sum_upper rate * z_k
-// The top non-empty level has no upper neighbours: its forward value is final.
+// s + sum over records k0..k1-1 of rate * w[neighbour] * y[neighbour] (the
+// forward factor's coefficients rate / pivot), loaded as in rec_sum
+template <typename Y>
+__device__ __forceinline__ double rec_sum_w(const lds_view_t& v, int k0, int k1, const Y* __restrict__ y,
+											const Y* __restrict__ w, double s)
+{
+	for (int k = k0; k < k1; k += 4)
+	{
+		uint32_t r[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+			r[i] = v.rec[k + i];
+		double rt[4];
+		Y yv[4], wv[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			rt[i] = v.rates[(r[i] >> 16) & 63];
+			yv[i] = y[r[i] & 0xffff];
+			wv[i] = w[r[i] & 0xffff];
+		}
+		__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			const double term = rt[i] * (double)wv[i] * (double)yv[i];
+			s += k + i < k1 ? term : 0.0;
+		}
+	}
+	return s;
+}
+
+// The factors of ILU(0) on these matrices.  There is no fill except on the
+// diagonal, so P = L1 U1 with L1 = I + L D~^{-1} and U1 = D~ + U, where L and
+// U are the strict lower and upper parts of M and D~ the pivots (iu = D~^{-1}).
+// In place on LDS, one workgroup barrier per non-empty level:
+//   lsolve: t <- L1^{-1} t,  y_p = t_p + sum_lower rate * iu_k y_k
+//           (the lowest non-empty level has no lower neighbours: skipped)
+//   usolve: t <- U1^{-1} t,  x_p = iu_p (t_p + sum_upper rate * x_k)
 // The bound of the next level is read while the current one is processed.
 template <typename P>
-__device__ void ilu_lds(int nl, const int* __restrict__ loff, const lds_view_t& v, const P* __restrict__ iu,
-						P* __restrict__ t)
+__device__ void lsolve_lds(int nl, const int* __restrict__ loff, const lds_view_t& v, const P* __restrict__ iu,
+						   P* __restrict__ t)
 {
 	const int tid = threadIdx.x;
 	int a = loff[0], e = loff[1];
-	int top = 0; // highest non-empty level
+	bool first = true;
 	for (int l = 0; l < nl; ++l)
 	{
 		const int e2 = loff[l + 2 <= nl ? l + 2 : nl];
 		if (a < e)
 		{
-			top = l;
-			for (int q = a + tid; q < e; q += pblock)
+			if (!first)
 			{
-				const uint32_t h = v.hdr[q];
-				const double tq = (double)t[q], iq = (double)iu[q];
-				const int k0 = hdr_first(h);
-				t[q] = (P)(iq * rec_sum<1>(v, k0, k0 + hdr_low(h), t, tq));
+				for (int q = a + tid; q < e; q += pblock)
+				{
+					const uint32_t h = v.hdr[q];
+					const double tq = (double)t[q];
+					const int k0 = hdr_first(h);
+					t[q] = (P)rec_sum_w(v, k0, k0 + hdr_low(h), t, iu, tq);
+				}
+				__syncthreads();
 			}
-			__syncthreads();
+			first = false;
 		}
 		a = e;
 		e = e2;
 	}
-	if (top == 0)
-		return;
-	a = loff[top - 1];
-	e = loff[top];
-	for (int l = top - 1; l >= 0; --l)
+}
+
+template <typename P>
+__device__ void usolve_lds(int nl, const int* __restrict__ loff, const lds_view_t& v, const P* __restrict__ iu,
+						   P* __restrict__ t)
+{
+	const int tid = threadIdx.x;
+	int a = loff[nl - 1], e = loff[nl];
+	for (int l = nl - 1; l >= 0; --l)
 	{
 		const int a2 = loff[l > 0 ? l - 1 : 0];
 		if (a < e)
@@ -1552,7 +1591,7 @@ __device__ void ilu_lds(int nl, const int* __restrict__ loff, const lds_view_t& 
 				const uint32_t h = v.hdr[q];
 				const double tq = (double)t[q], iq = (double)iu[q];
 				const int k0 = hdr_first(h);
-				t[q] = (P)(tq + iq * rec_sum<1>(v, k0 + hdr_low(h), k0 + hdr_count(h), t, 0.0));
+				t[q] = (P)(iq * rec_sum<1>(v, k0 + hdr_low(h), k0 + hdr_count(h), t, tq));
 			}
 			__syncthreads();
 		}
@@ -1898,7 +1937,29 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		// FP32 arithmetic can deliver); the FP64 residual decides what follows
 		const double inner_target = fmax(0.5 * tol, sizeof(T) == 4 ? inner_rtol * rel : 0.0);
 		++cycles;
-		const double inv_beta = 1.0 / beta;
+		// GMRES on the split-preconditioned system L1^{-1} M U1^{-1} (y = U1 x):
+		// the cycle starts from the preconditioned residual L1^{-1} r
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int p = tid + i * pblock;
+			if (p < nU)
+				s_t[p] = (P)ro[i];
+		}
+		__syncthreads();
+		lsolve_lds<P>(nl, s_loff, lv, s_iu, s_t);
+		double rp = 0.0;
+#pragma unroll
+		for (int i = 0; i < UPT; ++i)
+		{
+			const int p = tid + i * pblock;
+			ro[i] = p < nU ? (double)s_t[p] : 0.0;
+			rp += ro[i] * ro[i];
+		}
+		const double beta_p = sqrt(pg_sum(rp, s_red));
+		const double inv_beta = 1.0 / beta_p;
+		// the Arnoldi estimate |g| / beta_p scales the FP64 relative residual
+		const double est_scale = rel / beta_p;
 		T vj[UPT];
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
@@ -1912,12 +1973,14 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		}
 		// wavefront 0 applies the Givens rotations: lane i keeps rotation i of
 		// this cycle, g_run is the last entry of the rotated right-hand side
-		double g_run = beta, cs_l = 1.0, sn_l = 0.0;
+		double g_run = beta_p, cs_l = 1.0, sn_l = 0.0;
 
 		int k = 0;
 		for (int j = 0; j < m; ++j)
 		{
-			// t = P^{-1} v_j
+			// w = L1^{-1} M U1^{-1} v_j without a product by M (Eisenstat): with
+			// M = L1 D~ + U1 + (D - 2 D~),
+			//   x = U1^{-1} v,  w = D~ x + L1^{-1} (v + (D - 2 D~) x)
 #pragma unroll
 			for (int i = 0; i < UPT; ++i)
 			{
@@ -1926,14 +1989,28 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 					s_t[p] = (P)vj[i];
 			}
 			__syncthreads();
-			ilu_lds<P>(nl, s_loff, lv, s_iu, s_t);
-			// w = M t on the own rows
-			double w[UPT];
+			usolve_lds<P>(nl, s_loff, lv, s_iu, s_t);
+			double w[UPT]; // D~ x until the forward solve is added
 #pragma unroll
 			for (int i = 0; i < UPT; ++i)
 			{
 				const int p = tid + i * pblock;
-				w[i] = p < nU ? row_lds<P>(lv, p, dg[i], s_t) : 0.0;
+				w[i] = 0.0;
+				if (p < nU)
+				{
+					const double x = (double)s_t[p], dt = 1.0 / (double)s_iu[p];
+					w[i] = dt * x;
+					s_t[p] = (P)((double)vj[i] + (dg[i] - 2.0 * dt) * x);
+				}
+			}
+			__syncthreads();
+			lsolve_lds<P>(nl, s_loff, lv, s_iu, s_t);
+#pragma unroll
+			for (int i = 0; i < UPT; ++i)
+			{
+				const int p = tid + i * pblock;
+				if (p < nU)
+					w[i] += (double)s_t[p];
 			}
 			// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles
 			// (an FP32 basis is orthogonal to FP32 accuracy either way, and
@@ -2002,7 +2079,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 				if (lane == 0)
 				{
 					s_g[j] = cj * g_run;
-					s_scalar[1] = fabs(sj * g_run) / bn;
+					s_scalar[1] = fabs(sj * g_run) * est_scale;
 					s_scalar[2] = (hn > 0.0 && rn > 0.0) ? 1.0 / hn : 0.0;
 				}
 				g_run = -sj * g_run;
@@ -2033,7 +2110,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 			if (s_scalar[1] <= inner_target || scale == 0.0 || iters >= max_iters)
 				break;
 		}
-		// y = R^{-1} g;  x += P^{-1} (sum_i y_i V_i)
+		// y = R^{-1} g;  x += U1^{-1} (sum_i y_i V_i)
 		if (tid == 0)
 			for (int i = k - 1; i >= 0; --i)
 			{
@@ -2068,7 +2145,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 			}
 		}
 		__syncthreads();
-		ilu_lds<P>(nl, s_loff, lv, s_iu, s_t);
+		usolve_lds<P>(nl, s_loff, lv, s_iu, s_t);
 #pragma unroll
 		for (int i = 0; i < UPT; ++i)
 		{
