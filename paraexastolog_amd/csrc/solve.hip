@@ -1505,46 +1505,25 @@ __device__ __forceinline__ double rec_sum(const lds_view_t& v, int k0, int k1, c
 	return s;
 }
 
-// s + sum over records k0..k1-1 of rate * w[neighbour] * y[neighbour] (the
-// forward factor's coefficients rate / pivot), loaded as in rec_sum
-template <typename Y>
-__device__ __forceinline__ double rec_sum_w(const lds_view_t& v, int k0, int k1, const Y* __restrict__ y,
-											const Y* __restrict__ w, double s)
-{
-	for (int k = k0; k < k1; k += 4)
-	{
-		uint32_t r[4];
-#pragma unroll
-		for (int i = 0; i < 4; ++i)
-			r[i] = v.rec[k + i];
-		double rt[4];
-		Y yv[4], wv[4];
-#pragma unroll
-		for (int i = 0; i < 4; ++i)
-		{
-			rt[i] = v.rates[(r[i] >> 16) & 63];
-			yv[i] = y[r[i] & 0xffff];
-			wv[i] = w[r[i] & 0xffff];
-		}
-		__builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-		for (int i = 0; i < 4; ++i)
-		{
-			const double term = rt[i] * (double)wv[i] * (double)yv[i];
-			s += k + i < k1 ? term : 0.0;
-		}
-	}
-	return s;
-}
-
 // The factors of ILU(0) on these matrices.  There is no fill except on the
 // diagonal, so P = L1 U1 with L1 = I + L D~^{-1} and U1 = D~ + U, where L and
 // U are the strict lower and upper parts of M and D~ the pivots (iu = D~^{-1}).
 // In place on LDS, one workgroup barrier per non-empty level:
-//   lsolve: t <- L1^{-1} t,  y_p = t_p + sum_lower rate * iu_k y_k
-//           (the lowest non-empty level has no lower neighbours: skipped)
+//   lsolve: y = L1^{-1} t,  y_p = t_p + sum_lower rate * iu_k y_k, stored
+//           scaled, t_p <- iu_p y_p, so a neighbour costs one value load.  The
+//           lowest non-empty level has no lower neighbours and is skipped: its
+//           rows must hold iu_p t_p on entry (rows below lsolve_lowest(...)).
 //   usolve: t <- U1^{-1} t,  x_p = iu_p (t_p + sum_upper rate * x_k)
 // The bound of the next level is read while the current one is processed.
+// first row above the lowest non-empty level
+__device__ __forceinline__ int lsolve_lowest(int nl, const int* __restrict__ loff)
+{
+	for (int l = 0; l < nl; ++l)
+		if (loff[l] < loff[l + 1])
+			return loff[l + 1];
+	return loff[nl];
+}
+
 template <typename P>
 __device__ void lsolve_lds(int nl, const int* __restrict__ loff, const lds_view_t& v, const P* __restrict__ iu,
 						   P* __restrict__ t)
@@ -1562,9 +1541,9 @@ __device__ void lsolve_lds(int nl, const int* __restrict__ loff, const lds_view_
 				for (int q = a + tid; q < e; q += pblock)
 				{
 					const uint32_t h = v.hdr[q];
-					const double tq = (double)t[q];
+					const double tq = (double)t[q], iq = (double)iu[q];
 					const int k0 = hdr_first(h);
-					t[q] = (P)rec_sum_w(v, k0, k0 + hdr_low(h), t, iu, tq);
+					t[q] = (P)(iq * rec_sum<1>(v, k0, k0 + hdr_low(h), t, tq));
 				}
 				__syncthreads();
 			}
@@ -1891,6 +1870,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		__syncthreads();
 	}
 
+	const int lowest = lsolve_lowest(nl, s_loff);
 	int iters = 0, cycles = 0, stalled = 0;
 	double rel = 0.0, rel_prev = 0.0;
 	for (;;)
@@ -1944,7 +1924,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		{
 			const int p = tid + i * pblock;
 			if (p < nU)
-				s_t[p] = (P)ro[i];
+				s_t[p] = (P)(p < lowest ? (double)s_iu[p] * ro[i] : ro[i]);
 		}
 		__syncthreads();
 		lsolve_lds<P>(nl, s_loff, lv, s_iu, s_t);
@@ -1953,7 +1933,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		for (int i = 0; i < UPT; ++i)
 		{
 			const int p = tid + i * pblock;
-			ro[i] = p < nU ? (double)s_t[p] : 0.0;
+			ro[i] = p < nU ? (double)s_t[p] / (double)s_iu[p] : 0.0;
 			rp += ro[i] * ro[i];
 		}
 		const double beta_p = sqrt(pg_sum(rp, s_red));
@@ -1998,9 +1978,10 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 				w[i] = 0.0;
 				if (p < nU)
 				{
-					const double x = (double)s_t[p], dt = 1.0 / (double)s_iu[p];
+					const double x = (double)s_t[p], iq = (double)s_iu[p], dt = 1.0 / iq;
 					w[i] = dt * x;
-					s_t[p] = (P)((double)vj[i] + (dg[i] - 2.0 * dt) * x);
+					const double z = (double)vj[i] + (dg[i] - 2.0 * dt) * x;
+					s_t[p] = (P)(p < lowest ? iq * z : z);
 				}
 			}
 			__syncthreads();
@@ -2010,7 +1991,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 			{
 				const int p = tid + i * pblock;
 				if (p < nU)
-					w[i] += (double)s_t[p];
+					w[i] += (double)s_t[p] / (double)s_iu[p];
 			}
 			// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles
 			// (an FP32 basis is orthogonal to FP32 accuracy either way, and
