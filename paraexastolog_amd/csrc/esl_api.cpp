@@ -152,6 +152,14 @@ void esl_ctx_destroy(esl_ctx* ctx)
 	}
 	for (auto e : ctx->c.aux_ev)
 		(void)hipEventDestroy(e);
+	if (ctx->c.fill_s)
+	{
+		(void)hipStreamSynchronize(ctx->c.fill_s);
+		(void)hipStreamDestroy(ctx->c.fill_s);
+	}
+	for (auto e : ctx->c.fill_ev)
+		if (e)
+			(void)hipEventDestroy(e);
 	if (ctx->c.stream)
 		(void)hipStreamDestroy(ctx->c.stream);
 	delete ctx;

@@ -168,6 +168,9 @@ struct ctx_t
 	// forked streams (default priority) and fork/join events for plans solved concurrently
 	std::vector<hipStream_t> aux;
 	std::vector<hipEvent_t> aux_ev;
+	// stream (default priority) and fork/join events of a plan's final-vector fill
+	hipStream_t fill_s = nullptr;
+	hipEvent_t fill_ev[2] = {};
 	// captured sequence of the last concurrent multi-plan solve (owned by solve.hip)
 	void* multi = nullptr;
 	void (*multi_free)(void*) = nullptr;

@@ -3550,7 +3550,21 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 	ESL_HIP(hipGetLastError());
 }
 
-void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final)
+// the dense final vectors zeroed (algorithmic bytes: written once)
+void enqueue_final_fill(ctx_t& c, int B, int64_t n, double* d_final)
+{
+	timed_launch(c, "k_zero<final>", 8.0 * B * n, [&] {
+		const int64_t count = (int64_t)B * n;
+		if (count % 2 == 0 && ((uintptr_t)d_final & 15) == 0)
+			k_zero2<<<ceil_div(count / 2, sblock), sblock, 0, c.stream>>>(count / 2, (double2*)d_final);
+		else
+			k_zero<double><<<ceil_div(count, sblock), sblock, 0, c.stream>>>(count, d_final);
+	});
+}
+
+// clear: the final vectors still have to be zeroed here (false when the fill
+// ran on its own stream)
+void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, bool clear = true)
 {
 	graph_t& g = *P.g;
 	const int64_t n = g.n, n_t = g.n_t, T = g.T;
@@ -3574,14 +3588,8 @@ void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double
 	if (n_seg)
 		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, P.d_seg_off.get(), P.z.get(), P.invd.get(), n_seg,
 														  P.segsum.get());
-	// algorithmic bytes: the dense final vectors written once
-	timed_launch(c, "k_zero<final>", 8.0 * B * n, [&] {
-		const int64_t count = (int64_t)B * n;
-		if (count % 2 == 0 && ((uintptr_t)d_final & 15) == 0)
-			k_zero2<<<ceil_div(count / 2, sblock), sblock, 0, c.stream>>>(count / 2, (double2*)d_final);
-		else
-			k_zero<double><<<ceil_div(count, sblock), sblock, 0, c.stream>>>(count, d_final);
-	});
+	if (clear)
+		enqueue_final_fill(c, B, n, d_final);
 	if (n_t && Tr)
 		k_final<<<dim3(ceil_div(n_t, sblock), B), sblock, 0, c.stream>>>(
 			n, n_t, g.reordered.get(), g.rank.get(), P.d_jmap.get(), P.d_segmap.get(), P.pos.get(), P.z.get(),
@@ -3589,13 +3597,44 @@ void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double
 	ESL_HIP(hipGetLastError());
 }
 
-void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o)
+// fork_fill: the final-vector fill (HBM-write bound) runs on its own stream
+// beside the anchor and set-up kernels (LDS bound), and the solve waits for
+// it, so that the fill does not compete with the solve for HBM.
+void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
+				  bool fork_fill = false)
 {
+	const bool fork = fork_fill && P.nU > 0;
+	if (fork)
+	{
+		if (!c.fill_s)
+		{
+			ESL_HIP(hipStreamCreateWithFlags(&c.fill_s, hipStreamNonBlocking));
+			for (hipEvent_t& e : c.fill_ev)
+				ESL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		}
+		const hipStream_t main = c.stream;
+		ESL_HIP(hipEventRecord(c.fill_ev[0], main));
+		ESL_HIP(hipStreamWaitEvent(c.fill_s, c.fill_ev[0], 0));
+		c.stream = c.fill_s;
+		try
+		{
+			enqueue_final_fill(c, B, P.g->n, d_final);
+		}
+		catch (...)
+		{
+			c.stream = main;
+			throw;
+		}
+		c.stream = main;
+		ESL_HIP(hipEventRecord(c.fill_ev[1], c.fill_s));
+	}
 	plan_enqueue_setup(c, P, B, d_rates, o);
+	if (fork)
+		ESL_HIP(hipStreamWaitEvent(c.stream, c.fill_ev[1], 0));
 	if (P.nU)
 		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr, P.allow_lds,
 					   P.allow_reg);
-	plan_enqueue_post(c, P, B, d_rates, d_final);
+	plan_enqueue_post(c, P, B, d_rates, d_final, !fork);
 }
 
 void plan_reserve_(plan_t& P, int B, int m, int precision);
@@ -3676,7 +3715,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 				ESL_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
 				try
 				{
-					plan_enqueue(c, P, B, d_rates, d_final, o);
+					plan_enqueue(c, P, B, d_rates, d_final, o, true);
 				}
 				catch (...)
 				{
@@ -3699,7 +3738,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 				c.prof.book(P.gpend);
 		}
 		else
-			plan_enqueue(c, P, B, d_rates, d_final, o);
+			plan_enqueue(c, P, B, d_rates, d_final, o, true);
 		ESL_HIP(hipStreamSynchronize(c.stream));
 		if (P.nU > 0)
 		{
