@@ -133,6 +133,9 @@ def copy_calibration(L, ctx):
     print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), file=sys.stderr)
 
 
+SCHEDULE_NAMES = ["per-set LDS-resident", "per-set global", "device-wide", "per-set LDS, basis in memory"]
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -251,7 +254,7 @@ def main():
     restart = 32
     if a.schedule is not None:
         check(L.esl_plan_set_schedule(pl, a.schedule, 32))
-        schedule = {"path": ["per-set", "per-set global", "device-wide", "per-set LDS, basis in memory"][a.schedule],
+        schedule = {"path": SCHEDULE_NAMES[a.schedule],
                     "restart": 32, "pinned": True}
         restart = 0
     elif a.autotune:
@@ -261,7 +264,7 @@ def main():
         best = C.c_double()
         t = time.perf_counter()
         check(L.esl_plan_autotune(ctx.handle, pl, B, d_rates, d_final, C.byref(o), ch, C.byref(best)), ctx.handle)
-        schedule = {"path": ["per-set LDS-resident", "per-set global", "device-wide"][ch[0]], "restart": int(ch[1]),
+        schedule = {"path": SCHEDULE_NAMES[ch[0]], "restart": int(ch[1]),
                     "best_ms": best.value, "tuning_s": time.perf_counter() - t}
         restart = 0
     # kernel timing events on (they are captured into the HIP graph as event nodes)
@@ -459,7 +462,7 @@ def workload_n16(a, ctx):
     path, restart, best_ms = tp.plan.autotune(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1)
     sec, rep = _timed(lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1, restart=0), a.steps)
     out["autotuned"] = {"sets_per_s": B / sec, "ms_per_batch": 1000 * sec,
-                        "schedule": ["per-set LDS-resident", "per-set global", "device-wide"][path], "restart": restart}
+                        "schedule": SCHEDULE_NAMES[path], "restart": restart}
     best = max(v["sets_per_s"] for v in out.values())
     print(json.dumps({
         "metric": "rate sets/s, N=16 topology, batch of shared-plan rate sets", "value": best, "unit": "sets/s",

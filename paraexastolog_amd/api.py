@@ -392,7 +392,8 @@ class solve_plan:
               self.context.handle)
 
     def set_schedule(self, schedule, restart=32):
-        """Pins a schedule (0 per-set LDS, 1 per-set global, 2 device-wide; -1 forgets)."""
+        """Pins a schedule (0 per-set LDS, 1 per-set global, 2 device-wide, 3 per-set LDS with
+        the basis in global memory; -1 forgets)."""
         check(lib().esl_plan_set_schedule(self.handle, schedule, restart))
 
     def autotune(self, n_sets, d_rates, d_final, **opts):
