@@ -652,13 +652,20 @@ __global__ __launch_bounds__(sblock) void k_zero(int64_t count, T* __restrict__ 
 		p[i] = (T)0;
 }
 
-// the dense final vectors: one 16-byte store per thread (6.9 TB/s on MI355X
-// against 6.4 with 8-byte stores; grid-stride and non-temporal forms were slower)
+// the dense final vectors: one 16-byte non-temporal store per thread.  The
+// fill runs beside the solve kernel, whose spilled Krylov basis lives in the
+// caches: streaming stores leave the caches to it (N=20 bench, fill and solve
+// overlapped: 2.96 -> 2.46 ms per step).  16-byte stores beat 8-byte ones
+// (6.9 against 6.4 TB/s alone); grid-stride loops were slower.
+typedef double zero2_t __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(sblock) void k_zero2(int64_t count2, double2* __restrict__ p)
 {
 	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	if (i < count2)
-		p[i] = make_double2(0.0, 0.0);
+	{
+		const zero2_t z = { 0.0, 0.0 };
+		__builtin_nontemporal_store(z, (zero2_t*)p + i);
+	}
 }
 
 // ============================================================================
