@@ -46,6 +46,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <memory>
 
 #include "common.hip.hpp"
@@ -2977,6 +2978,52 @@ struct krylov_result_t
 	int64_t iters_sum = 0; // iterations summed over the sets
 };
 
+// device scratch of the per-set GMRES kernels (pointers only)
+struct pss_view_t
+{
+	double *r = nullptr, *udiag = nullptr, *res = nullptr, *t = nullptr;
+	int32_t *its = nullptr, *cyc = nullptr, *stl = nullptr;
+	float* Vf = nullptr;  // FP32 basis (precision 1)
+	double* Vd = nullptr; // FP64 basis (precision 0, and the FP64 finish of precision 1)
+};
+
+std::string sci(double v)
+{
+	char buf[32];
+	std::snprintf(buf, sizeof buf, "%.3e", v);
+	return buf;
+}
+
+// per-set results of the last launch: max / sums over the sets, and the host
+// copies of the per-set residuals, iteration and cycle counts
+krylov_result_t collect_sets(ctx_t& c, const pss_view_t& sc, int B, double tol, bool check,
+							 std::vector<double>* h_res_out = nullptr, std::vector<int32_t>* h_its_out = nullptr)
+{
+	krylov_result_t kr;
+	std::vector<int32_t> h_its, h_cyc;
+	std::vector<double> h_res;
+	read_host(c, h_its, sc.its, B);
+	read_host(c, h_cyc, sc.cyc, B);
+	read_host(c, h_res, sc.res, B);
+	for (int b = 0; b < B; ++b)
+	{
+		kr.resid = std::max(kr.resid, std::isnan(h_res[b]) ? 1e300 : h_res[b]);
+		kr.iters = std::max(kr.iters, h_its[b]);
+		kr.cycles = std::max(kr.cycles, h_cyc[b]);
+		kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
+		kr.missed += (h_res[b] <= tol) ? 0 : 1;
+		kr.iters_sum += h_its[b];
+	}
+	if (check && !(kr.resid <= tol))
+		throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " + sci(kr.resid) +
+									   " after " + std::to_string(kr.iters) + " iterations");
+	if (h_res_out)
+		*h_res_out = std::move(h_res);
+	if (h_its_out)
+		*h_its_out = std::move(h_its);
+	return kr;
+}
+
 // scratch of the per-set GMRES kernel, grown on demand and reused
 struct per_set_scratch_t
 {
@@ -2999,43 +3046,23 @@ struct per_set_scratch_t
 		else
 			Vd.ensure((size_t)B * (m + 1) * nU);
 	}
-	krylov_result_t collect(ctx_t& c, int B, double tol, bool any, bool check = true)
+	pss_view_t view() const
 	{
-		krylov_result_t kr;
-		if (!any)
-			return kr;
-		std::vector<int32_t> h_its, h_cyc;
-		std::vector<double> h_res;
-		read_host(c, h_its, its.get(), B);
-		read_host(c, h_cyc, cyc.get(), B);
-		read_host(c, h_res, res.get(), B);
-		for (int b = 0; b < B; ++b)
-		{
-			kr.resid = std::max(kr.resid, std::isnan(h_res[b]) ? 1e300 : h_res[b]);
-			kr.iters = std::max(kr.iters, h_its[b]);
-			kr.cycles = std::max(kr.cycles, h_cyc[b]);
-			kr.sets_iterated += h_cyc[b] > 0 ? 1 : 0;
-			kr.missed += (h_res[b] <= tol) ? 0 : 1;
-			kr.iters_sum += h_its[b];
-		}
-		if (check && !(kr.resid <= tol))
-			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-										   std::to_string(kr.resid) + " after " + std::to_string(kr.iters) +
-										   " iterations");
-		return kr;
+		return pss_view_t{ r.get(), udiag.get(), res.get(), t.get(), its.get(), cyc.get(), stl.get(),
+						   Vf.get(), Vd.get() };
 	}
 };
 
 // one launch: GMRES of every set, one workgroup per set (no host synchronisation)
 void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs, double* z,
-					const esl_solve_opts& o, per_set_scratch_t& sc, bool allow_lds = true, bool allow_reg = true)
+					const esl_solve_opts& o, const pss_view_t& sc, bool allow_lds = true, bool allow_reg = true)
 {
 	const int B = S.B;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
 	const int nseg1 = std::max(n_seg, 1);
 	const bool ilu = o.preconditioner != 1; // ILU(0) unless Jacobi is asked for
 	const nbr_t G = nb.view();
-	pgm_out_t po{ sc.its.get(), sc.cyc.get(), sc.res.get(), sc.stl.get() };
+	pgm_out_t po{ sc.its, sc.cyc, sc.res, sc.stl };
 	const double by = gmres_set_bytes(S, nb.nnz);
 	if (allow_lds && ilu && nb.packed)
 	{
@@ -3057,7 +3084,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			}
 			if (o.precision == 1 && !small && m <= 32 && allow_reg)
 			{
-				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vf };
 				timed_launch(c, "k_gmres_priv8<float>", by, [&] {
 					if (m <= 16)
 						k_gmres_priv8<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -3073,7 +3100,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			else if (o.precision == 1 && small && m <= 32 && allow_reg)
 			{
 				// the FP32 basis of each thread's rows as a private array
-				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vf };
 				timed_launch(c, "k_gmres_priv<float>", by, [&] {
 					if (m <= 16)
 						k_gmres_priv<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -3088,7 +3115,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			}
 			else if (o.precision == 1)
 			{
-				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+				pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vf };
 				timed_launch(c, "k_gmres_lds<float>", by, [&] {
 					if (small)
 						k_gmres_lds<float, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -3100,7 +3127,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			}
 			else
 			{
-				pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vd.get() };
+				pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vd };
 				timed_launch(c, "k_gmres_lds<double>", by, [&] {
 					if (small)
 						k_gmres_lds<double, 4><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
@@ -3116,7 +3143,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	}
 	if (o.precision == 1)
 	{
-		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vf.get() };
+		pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vf };
 		timed_launch(c, ilu ? "k_gmres_set<float,ilu>" : "k_gmres_set<float,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<float, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
@@ -3128,7 +3155,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	}
 	else
 	{
-		pgm_scratch_t sp{ sc.r.get(), sc.udiag.get(), sc.t.get(), sc.Vd.get() };
+		pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vd };
 		timed_launch(c, ilu ? "k_gmres_set<double,ilu>" : "k_gmres_set<double,jacobi>", by, [&] {
 			if (ilu)
 				k_gmres_set<double, true><<<B, pblock, 0, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters,
@@ -3145,19 +3172,21 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 // missed the tolerance (their FP32 cycles stalled) are finished by the FP64
 // kernel from their current iterate; sets that already converged leave it at
 // their first residual check.
+// fp64_view() gives the scratch with an FP64 basis (it may grow it).
+template <typename F>
 krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs,
-							   double* z, esl_solve_opts o, per_set_scratch_t& sc)
+							   double* z, esl_solve_opts o, const pss_view_t& sc, F&& fp64_view)
 {
 	ESL_HIP(hipStreamSynchronize(c.stream));
 	const int B = S.B;
-	krylov_result_t kr = sc.collect(c, B, o.tol, true, o.precision != 1);
+	krylov_result_t kr = collect_sets(c, sc, B, o.tol, o.precision != 1);
 	if (o.precision != 1 || kr.missed == 0)
 		return kr;
 	o.precision = 0;
-	sc.reserve(B, S.nU, o.restart, 0);
-	launch_per_set(c, S, nb, n_seg, rhs, z, o, sc);
+	const pss_view_t sd = fp64_view();
+	launch_per_set(c, S, nb, n_seg, rhs, z, o, sd);
 	ESL_HIP(hipStreamSynchronize(c.stream));
-	const krylov_result_t k2 = sc.collect(c, B, o.tol, true, true);
+	const krylov_result_t k2 = collect_sets(c, sd, B, o.tol, true);
 	kr.fp64_fallback = kr.missed;
 	kr.missed = 0;
 	kr.resid = k2.resid;
@@ -3185,6 +3214,20 @@ struct plan_t
 	dbuf<double> p0, p1, invd, rhs, z, y, segsum, fpart;
 	dbuf<int32_t> anchor;
 	per_set_scratch_t scr;
+	// the buffers the run being enqueued uses: the ones above (plan_reserve), or
+	// slices of the library's static scratch (plan_run_noalloc)
+	struct run_bufs_t
+	{
+		double *p0 = nullptr, *p1 = nullptr, *invd = nullptr, *rhs = nullptr, *z = nullptr, *y = nullptr,
+			   *segsum = nullptr, *fpart = nullptr;
+		int32_t* anchor = nullptr;
+		pss_view_t ps;
+	} rb;
+	void use_own_bufs()
+	{
+		rb = run_bufs_t{ p0.get(), p1.get(), invd.get(), rhs.get(), z.get(), y.get(), segsum.get(), fpart.get(),
+						 anchor.get(), scr.view() };
+	}
 	int anchor_sweeps = 32;
 	size_t anchor_lds = 0; // dynamic LDS of k_anchor_lds, 0 when the terminal unknowns do not fit
 	size_t anchor_lvl = 0; // dynamic LDS of k_anchor_lvl, 0 without packed records
@@ -3265,8 +3308,11 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		o.tol = tol;
 		o.max_iters = max_iters;
 		o.restart = m;
-		launch_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc);
-		kr = finish_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc);
+		launch_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc.view());
+		kr = finish_per_set(c, S, nb, n_seg, rhs_p, z_p, o, sc.view(), [&] {
+			sc.reserve(B, nU, m, 0);
+			return sc.view();
+		});
 	}
 	else
 	{
@@ -3507,8 +3553,8 @@ sys_t plan_t::sys(int B, const double* rates) const
 	S.rates = rates;
 	S.rate_stride = 2 * g->N;
 	S.transpose = 0;
-	S.anchor = anchor.get();
-	S.invd = invd.get();
+	S.anchor = rb.anchor;
+	S.invd = rb.invd;
 	return S;
 }
 
@@ -3528,17 +3574,17 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 		const dim3 gt(ceil_div(nUt, sblock), B);
 		if (warm)
 			// the prediction picks the anchors (no Jacobi sweeps)
-			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, P.d_tpos.get(), o.warm_pi, P.p0.get());
+			k_warm_gather<<<gt, sblock, 0, c.stream>>>(nUt, n_t, P.d_tpos.get(), o.warm_pi, P.rb.p0);
 		else if (P.anchor_lvl)
 		{
 			// sweeps and pick in one launch on the solver's level-ordered records
 			timed_launch(c, "k_anchor_lvl", 8.0 * B * nUt, [&] {
 				if (nU <= 4 * pblock)
 					k_anchor_lvl<4><<<B, pblock, P.anchor_lvl, c.stream>>>(S, P.nbr.view(), P.d_seg_off.get(), n_seg,
-																		   P.anchor_sweeps, P.anchor.get());
+																		   P.anchor_sweeps, P.rb.anchor);
 				else
 					k_anchor_lvl<8><<<B, pblock, P.anchor_lvl, c.stream>>>(S, P.nbr.view(), P.d_seg_off.get(), n_seg,
-																		   P.anchor_sweeps, P.anchor.get());
+																		   P.anchor_sweeps, P.rb.anchor);
 			});
 		}
 		else if (P.anchor_lds)
@@ -3547,29 +3593,32 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 			timed_launch(c, "k_anchor_lds", 8.0 * B * nUt, [&] {
 				k_anchor_lds<<<B, sblock, P.anchor_lds, c.stream>>>(S, P.nbr.ptr.get(), P.nbr.nu.get(),
 																	 P.nbr.nr.get(), P.d_seg_off.get(), n_seg,
-																	 P.anchor_sweeps, P.anchor.get());
+																	 P.anchor_sweeps, P.rb.anchor);
 			});
 		}
 		else
 		{
 			k_fill_value<<<ceil_div((int64_t)B * nUt, sblock), sblock, 0, c.stream>>>((int64_t)B * nUt, 1.0,
-																					  P.p0.get());
+																					  P.rb.p0);
+			double *pa = P.rb.p0, *pb = P.rb.p1;
 			for (int it = 0; it < P.anchor_sweeps; ++it)
 			{
-				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, P.p0.get(), P.p1.get());
-				std::swap(P.p0, P.p1);
+				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, pa, pb);
+				std::swap(pa, pb);
 			}
+			if (pa != P.rb.p0)
+				std::swap(P.rb.p0, P.rb.p1); // the mass after the last sweep
 		}
 		if (warm || (!P.anchor_lds && !P.anchor_lvl))
-			k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.p0.get(), n_seg,
-																	 P.anchor.get());
+			k_anchor_pick<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nUt, P.d_seg_off.get(), P.rb.p0, n_seg,
+																	 P.rb.anchor);
 	}
 	if (nU)
 	{
 		const dim3 gu(ceil_div(nU, sblock), B);
-		k_sys_setup<<<gu, sblock, 0, c.stream>>>(S, nseg1, P.x0.get(), P.invd.get(), P.rhs.get(), P.z.get());
+		k_sys_setup<<<gu, sblock, 0, c.stream>>>(S, nseg1, P.x0.get(), P.rb.invd, P.rb.rhs, P.rb.z);
 		if (warm)
-			k_warm_z<<<dim3(ceil_div(nUt, sblock), B), sblock, 0, c.stream>>>(S, n_seg, P.p0.get(), P.z.get());
+			k_warm_z<<<dim3(ceil_div(nUt, sblock), B), sblock, 0, c.stream>>>(S, n_seg, P.rb.p0, P.rb.z);
 	}
 	ESL_HIP(hipGetLastError());
 }
@@ -3601,23 +3650,23 @@ void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double
 		{
 			const int nbA = ceil_div(P.nA, sblock);
 			k_exit_flow<<<dim3(nbA, B), sblock, Tr * sizeof(double), c.stream>>>(S, g.rank.get(), T, P.d_jmap.get(),
-																				 P.tmask.get(), P.z.get(), Tr,
-																				 P.fpart.get());
-			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(P.fpart.get(), nbA, Tr, P.y.get(), Tr, 0);
+																				 P.tmask.get(), P.rb.z, Tr,
+																				 P.rb.fpart);
+			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(P.rb.fpart, nbA, Tr, P.rb.y, Tr, 0);
 		}
 		else
-			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, P.y.get());
-		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, P.y0.get(), P.y.get());
+			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, P.rb.y);
+		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, P.y0.get(), P.rb.y);
 	}
 	if (n_seg)
-		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, P.d_seg_off.get(), P.z.get(), P.invd.get(), n_seg,
-														  P.segsum.get());
+		k_seg_sum<<<dim3(n_seg, B), sblock, 0, c.stream>>>(nU, P.d_seg_off.get(), P.rb.z, P.rb.invd, n_seg,
+														  P.rb.segsum);
 	if (clear)
 		enqueue_final_fill(c, B, n, d_final);
 	if (n_t && Tr)
 		k_final<<<dim3(ceil_div(n_t, sblock), B), sblock, 0, c.stream>>>(
-			n, n_t, g.reordered.get(), g.rank.get(), P.d_jmap.get(), P.d_segmap.get(), P.pos.get(), P.z.get(),
-			P.invd.get(), P.segsum.get(), nseg1, nU, P.y.get(), Tr, d_final);
+			n, n_t, g.reordered.get(), g.rank.get(), P.d_jmap.get(), P.d_segmap.get(), P.pos.get(), P.rb.z,
+			P.rb.invd, P.rb.segsum, nseg1, nU, P.rb.y, Tr, d_final);
 	ESL_HIP(hipGetLastError());
 }
 
@@ -3655,7 +3704,7 @@ void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_f
 	}
 	plan_enqueue_setup(c, P, B, d_rates, o);
 	if (P.nU)
-		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), o, P.scr, P.allow_lds,
+		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, o, P.rb.ps, P.allow_lds,
 					   P.allow_reg);
 	if (fork)
 		ESL_HIP(hipStreamWaitEvent(c.stream, c.fill_ev[1], 0));
@@ -3694,6 +3743,7 @@ void plan_reserve_(plan_t& P, int B, int m, int precision)
 	P.segsum.ensure((size_t)B * std::max(P.n_seg, 1));
 	P.fpart.ensure((size_t)B * std::max(ceil_div(std::max<int64_t>(P.nA, 1), sblock), 1) * std::max(P.Tr, 1));
 	P.scr.reserve(B, nU, m, precision);
+	P.use_own_bufs();
 }
 
 void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
@@ -3769,7 +3819,11 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		{
 			esl_solve_opts of = o;
 			of.restart = m;
-			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), of, P.scr);
+			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, of, P.rb.ps, [&] {
+				P.scr.reserve(B, P.nU, m, 0);
+				P.use_own_bufs();
+				return P.rb.ps;
+			});
 			if (kr.fp64_fallback) // the final vectors are rebuilt from the FP64 iterates
 				plan_enqueue_post(c, P, B, d_rates, d_final);
 			ESL_HIP(hipStreamSynchronize(c.stream));
@@ -3780,7 +3834,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		// one large system: device-wide GMRES (host-driven restarts)
 		plan_reserve(P, B, 2, 0);
 		plan_enqueue_setup(c, P, B, d_rates, o);
-		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rhs.get(), P.z.get(), o, &P.nbr, true);
+		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rb.rhs, P.rb.z, o, &P.nbr, true);
 		plan_enqueue_post(c, P, B, d_rates, d_final);
 		ESL_HIP(hipStreamSynchronize(c.stream));
 	}
@@ -4004,7 +4058,11 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		{
 			const esl_solve_opts& of = po[i];
 			const krylov_result_t kr =
-				finish_per_set(c, P.sys(B[i], d_rates[i]), P.nbr, P.n_seg, P.rhs.get(), P.z.get(), of, P.scr);
+				finish_per_set(c, P.sys(B[i], d_rates[i]), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, of, P.rb.ps, [&] {
+					P.scr.reserve(B[i], P.nU, of.restart, 0);
+					P.use_own_bufs();
+					return P.rb.ps;
+				});
 			if (kr.fp64_fallback)
 				plan_enqueue_post(c, P, B[i], d_rates[i], d_final[i]);
 			rep.residual = kr.resid;
