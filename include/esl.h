@@ -223,6 +223,22 @@ esl_status esl_plan_query(const esl_plan* p, int64_t info[8]);
 esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 						  const esl_solve_opts* opts, esl_solve_report* report);
 
+/* esl_plan_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * solve (tight serving loops, harnesses that forbid hipMalloc).  Every per-run
+ * buffer (anchors, coefficients, iterates, Krylov basis, partial sums) is a
+ * slice of one static device array of 64 MiB that the library holds on each
+ * GPU until the process ends; the caller passes the plan (made beforehand by
+ * esl_plan_create) and the input and output buffers as for esl_plan_solve.
+ * A batch whose buffers exceed 64 MiB (about 200 rate sets of a 1,000-unknown
+ * system at restart 32), or a plan that needs the device-wide solver, is
+ * ESL_E_ARG with the needed size in the message.  opts.graph is ignored (no
+ * capture) and kernel profiling is paused during the call.  Calls on one GPU
+ * share the array: calls that overlap in time on one GPU (two threads, two
+ * contexts) race; each call writes every slice it reads before reading it. */
+esl_status esl_plan_solve_noalloc(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+								  const esl_solve_opts* opts, esl_solve_report* report);
+
 /* d_out[n_sets][n_t] = d_final[set][reordered_vertices[i]] for i < n_t: the final
  * vectors on the terminal states in canonical order -- the layout of opts->warm_pi
  * and of the surrogate's targets (include/esl_mlp.h).  No reference counterpart. */

@@ -108,6 +108,7 @@ SIGNATURES = [
     ("esl_plan_destroy", None, [_VP]),
     ("esl_plan_query", C.c_int, [_VP, _P(C.c_int64)]),
     ("esl_plan_solve", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
+    ("esl_plan_solve_noalloc", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
     ("esl_plan_gather_terminal", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_plan_set_schedule", C.c_int, [_VP, C.c_int, C.c_int]),
     ("esl_plan_autotune", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(C.c_int32), _P(C.c_double)]),

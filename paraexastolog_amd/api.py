@@ -386,6 +386,18 @@ class solve_plan:
                                    C.byref(rep)), self.context.handle)
         return rep.as_dict()
 
+    def solve_noalloc(self, n_sets, d_rates, d_final, **opts):
+        """As solve(), but every per-run buffer is a slice of the library's static
+        device scratch (esl_plan_solve_noalloc: no device allocation; 64 MiB)."""
+        o = solve_opts(**{k: v for k, v in opts.items() if k not in ("graph", "warm")})
+        warm = opts.get("warm")
+        if warm is not None:
+            o.warm_pi = C.cast(warm, C.POINTER(C.c_double))
+        rep = SolveReport()
+        check(lib().esl_plan_solve_noalloc(self.context.handle, self.handle, n_sets, d_rates, d_final, C.byref(o),
+                                           C.byref(rep)), self.context.handle)
+        return rep.as_dict()
+
     def gather_terminal(self, n_sets, d_final, d_out):
         """d_out[n_sets][n_t] = final vectors on the terminal states (warm-start layout)."""
         check(lib().esl_plan_gather_terminal(self.context.handle, self.handle, n_sets, d_final, d_out),

@@ -124,6 +124,7 @@ esl_status esl_ctx_create(int device, esl_ctx** out)
 			ESL_HIP(hipEventCreate(&e));
 		ctx->c.flags.ensure(64);
 		ESL_HIP(hipHostMalloc((void**)&ctx->c.h_flags, 64 * sizeof(int32_t), hipHostMallocDefault));
+		esl::noalloc_init(ctx->c);
 		*out = ctx.release();
 	});
 }
@@ -611,6 +612,28 @@ esl_status esl_plan_solve(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d
 		require(!o.full, ESL_E_ARG, "opts.full is not available on plans; use esl_solve");
 		esl_solve_report rep{};
 		esl::plan_solve(ctx->c, p->p, n_sets, d_rates, d_final, o, rep);
+		if (report)
+			*report = rep;
+	});
+}
+
+esl_status esl_plan_solve_noalloc(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
+								  const esl_solve_opts* opts, esl_solve_report* report)
+{
+	return guarded(ctx, [&] {
+		require(ctx && p && d_rates && d_final, ESL_E_ARG, "null argument");
+		require(p->ctx == ctx, ESL_E_ARG, "plan belongs to another context");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		set_device(ctx);
+		esl_solve_opts o;
+		esl_solve_opts_default(&o);
+		if (opts)
+			o = *opts;
+		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		require(!o.full, ESL_E_ARG, "opts.full is not available on plans; use esl_solve");
+		o.graph = 0; // never captured by the library
+		esl_solve_report rep{};
+		esl::plan_solve_noalloc(ctx->c, p->p, n_sets, d_rates, d_final, o, rep);
 		if (report)
 			*report = rep;
 	});

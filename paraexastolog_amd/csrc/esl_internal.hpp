@@ -174,7 +174,14 @@ struct ctx_t
 	// captured sequence of the last concurrent multi-plan solve (owned by solve.hip)
 	void* multi = nullptr;
 	void (*multi_free)(void*) = nullptr;
+	// device address of the library's static scratch array on this device (noalloc_init)
+	void* noalloc_scratch = nullptr;
 };
+
+// size of the static device scratch behind esl_plan_solve_noalloc, and the
+// device indices it keeps an address for
+constexpr size_t noalloc_scratch_bytes = (size_t)64 << 20;
+constexpr int max_devices = 64;
 
 // Appends an event-record node to the graph being captured on `s`, after the
 // nodes captured so far, and makes it the dependency of what follows.
@@ -292,6 +299,10 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 				   int32_t choice[2], double* best_ms);
 void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final, double* d_out);
 void plan_set_schedule(void* p, int path, int restart);
+// the plan solve whose buffers are slices of the static scratch (no allocation)
+void noalloc_init(ctx_t& c); // once per device, with that device current
+void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
+						esl_solve_report& rep);
 // the implementation object behind an esl_ctx handle (defined in esl_api.cpp)
 ctx_t& ctx_impl(esl_ctx* ctx);
 // solution_term / solution_nonterm of one rate set into g.sol (reference layout)

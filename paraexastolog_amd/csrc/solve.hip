@@ -48,6 +48,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 
 #include "common.hip.hpp"
 #include "esl_internal.hpp"
@@ -3873,6 +3874,157 @@ void plan_solve(ctx_t& c, void* p, int B, const double* d_rates, double* d_final
 				esl_solve_report& rep)
 {
 	plan_run(c, *static_cast<plan_t*>(p), B, d_rates, d_final, o, rep);
+}
+
+// ---- the solve that allocates nothing: every per-run buffer of a plan is a
+// slice of one static device array.  Each GPU the library runs on holds its own
+// copy of the array until the process ends; calls on one GPU share it (and
+// race when they overlap), and each call rewrites what it reads.
+
+static __device__ __align__(256) unsigned char scratch[noalloc_scratch_bytes];
+
+namespace {
+std::mutex noalloc_mu;
+void* noalloc_dev_ptr[max_devices] = {}; // device address of `scratch`, per device
+} // namespace
+
+void noalloc_init(ctx_t& c)
+{
+	std::lock_guard<std::mutex> lk(noalloc_mu);
+	if (c.device < 0 || c.device >= max_devices)
+		throw error(ESL_E_ARG, "device index " + std::to_string(c.device) + " is beyond the library's table");
+	if (!noalloc_dev_ptr[c.device])
+	{
+		void* ptr = nullptr;
+		ESL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch))); // the calling thread has c.device current
+		noalloc_dev_ptr[c.device] = ptr;
+	}
+	c.noalloc_scratch = noalloc_dev_ptr[c.device];
+}
+
+namespace {
+// bump allocator over [base, base + cap): 256-byte aligned slices
+struct carve_t
+{
+	uintptr_t base, at;
+	template <typename T>
+	T* take(size_t count)
+	{
+		at = (at + 255) & ~(uintptr_t)255;
+		T* p = reinterpret_cast<T*>(at);
+		at += std::max<size_t>(count, 1) * sizeof(T);
+		return p;
+	}
+	size_t used() const { return (size_t)(at - base); }
+};
+
+// the per-run buffers of B sets at restart m; the basis is sized for FP64
+// (precision 1 keeps its FP32 basis in the same slice and reuses it for the
+// FP64 finish of stalled sets)
+plan_t::run_bufs_t carve_run_bufs(const plan_t& P, int B, int m, carve_t& cv)
+{
+	const size_t nU = (size_t)std::max<int64_t>(P.nU, 1), nUt = (size_t)std::max<int64_t>(P.nUt, 1);
+	const size_t nseg = (size_t)std::max(P.n_seg, 1), Tr = (size_t)std::max(P.Tr, 1);
+	const size_t nbA = (size_t)std::max(ceil_div(std::max<int64_t>(P.nA, 1), sblock), 1);
+	plan_t::run_bufs_t rb;
+	rb.p0 = cv.take<double>(B * nUt);
+	rb.p1 = cv.take<double>(B * nUt);
+	rb.anchor = cv.take<int32_t>(B * nseg);
+	rb.invd = cv.take<double>(B * nU);
+	rb.rhs = cv.take<double>(B * nU);
+	rb.z = cv.take<double>(B * nU);
+	rb.y = cv.take<double>(B * Tr);
+	rb.segsum = cv.take<double>(B * nseg);
+	rb.fpart = cv.take<double>(B * nbA * Tr);
+	rb.ps.r = cv.take<double>(B * nU);
+	rb.ps.udiag = cv.take<double>(B * nU);
+	rb.ps.t = cv.take<double>(B * nU);
+	rb.ps.res = cv.take<double>(B);
+	rb.ps.its = cv.take<int32_t>(B);
+	rb.ps.cyc = cv.take<int32_t>(B);
+	rb.ps.stl = cv.take<int32_t>(B);
+	rb.ps.Vd = cv.take<double>((size_t)B * (m + 1) * nU);
+	rb.ps.Vf = reinterpret_cast<float*>(rb.ps.Vd);
+	return rb;
+}
+
+// profiling off for the call (its event pool would grow), restored on exit
+struct prof_pause_t
+{
+	ctx_t& c;
+	bool was;
+	explicit prof_pause_t(ctx_t& x) : c(x), was(x.prof.on) { c.prof.on = false; }
+	~prof_pause_t() { c.prof.on = was; }
+};
+} // namespace
+
+void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
+						esl_solve_report& rep)
+{
+	plan_t& P = *static_cast<plan_t*>(p);
+	if (!c.noalloc_scratch)
+		throw error(ESL_E_STATE, "the static scratch of this context's device was not set up");
+	esl_solve_opts o = o_in;
+	if (o.tol <= 0)
+		o.tol = 1e-13;
+	if (o.max_iters <= 0)
+		o.max_iters = 20000;
+	const bool tuned = o.restart == 0 && P.tune.set;
+	if (tuned)
+		o.restart = P.tune.m;
+	const bool allow_lds = !(tuned && P.tune.path == 1);
+	const bool allow_reg = !(tuned && P.tune.path == 3);
+	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
+	o.restart = m;
+	const bool per_set = P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B));
+	if (!per_set)
+		throw error(ESL_E_ARG, "the no-allocation solve runs the per-set solvers only; this plan's system (" +
+								   std::to_string(P.nU) + " unknowns per set) needs the device-wide solver");
+	carve_t dry{ 0, 0 };
+	(void)carve_run_bufs(P, B, m, dry);
+	if (dry.used() > noalloc_scratch_bytes)
+		throw error(ESL_E_ARG, std::to_string(B) + " rate sets of this plan at restart " + std::to_string(m) +
+								   " need " + std::to_string(dry.used()) + " bytes of scratch; the static scratch holds " +
+								   std::to_string(noalloc_scratch_bytes) + " (use fewer sets per call)");
+	const uintptr_t base = reinterpret_cast<uintptr_t>(c.noalloc_scratch);
+	carve_t cv{ base, base };
+	prof_pause_t pause(c);
+	P.rb = carve_run_bufs(P, B, m, cv);
+	P.allow_lds = allow_lds;
+	P.allow_reg = allow_reg;
+	rep.terminals_reached = P.Tr;
+	rep.unknowns = P.nU;
+	rep.reachable = -1;
+	krylov_result_t kr;
+	try
+	{
+		ESL_HIP(hipEventRecord(c.ev[0], c.stream));
+		plan_enqueue(c, P, B, d_rates, d_final, o, false);
+		if (P.nU > 0)
+		{
+			const pss_view_t ps = P.rb.ps;
+			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, o, ps, [&] { return ps; });
+			if (kr.fp64_fallback) // the final vectors are rebuilt from the FP64 iterates
+				plan_enqueue_post(c, P, B, d_rates, d_final);
+		}
+		ESL_HIP(hipEventRecord(c.ev[1], c.stream));
+		ESL_HIP(hipEventSynchronize(c.ev[1]));
+		float ms = 0;
+		ESL_HIP(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+		rep.ms_stage[3] = rep.ms_total = ms;
+	}
+	catch (...)
+	{
+		P.use_own_bufs();
+		throw;
+	}
+	P.use_own_bufs(); // pointers only: later runs of the plan use its own buffers again
+	rep.residual = kr.resid;
+	rep.iterations = kr.iters;
+	rep.refinements = kr.cycles;
+	rep.sets_iterated = kr.sets_iterated;
+	rep.fp64_fallback = kr.fp64_fallback;
+	rep.iterations_sum = kr.iters_sum;
 }
 
 // ---- several plans (distinct networks) in one call: their per-set solves run
