@@ -279,6 +279,42 @@ def terminal_kernel_det(Kj):
     return m / m.sum()
 
 
+def vertex_heights(n, indptr, rows, ids):
+    """Height of every vertex's SCC (1 for terminal SCCs)."""
+    roots, h = scc_heights(n, indptr, rows, ids)
+    pos = np.full(n, -1, dtype=np.int64)
+    pos[roots] = np.arange(len(roots))
+    return h[pos[ids]]
+
+
+def solve_by_levels(A, b, hv):
+    """Solves A w = b for a generator block A over transient vertices listed in
+    canonical order (heights `hv` non-increasing): an entry A[d, s] (s -> d)
+    joins two SCCs only when height(s) > height(d), so A is block lower
+    triangular by height level and, inside a level, block diagonal by SCC.
+    Forward substitution level by level, one SuperLU factorisation per level
+    (its fill stays inside each SCC).  Exact up to rounding, like one sparse LU
+    of A, without the fill a global ordering leaves across SCCs."""
+    A = sp.csr_matrix(A)
+    m = len(b)
+    w = np.zeros(m)
+    if m == 0:
+        return w
+    cuts = np.flatnonzero(np.diff(hv)) + 1
+    starts = np.r_[0, cuts]
+    ends = np.r_[cuts, m]
+    for a, e in zip(starts, ends):
+        r = np.array(b[a:e], dtype=float)
+        if a > 0:
+            r -= A[a:e, :a] @ w[:a]
+        D = A[a:e, a:e]
+        if e - a == 1:
+            w[a] = r[0] / D[0, 0]
+        else:
+            w[a:e] = sla.splu(D.tocsc()).solve(r)
+    return w
+
+
 class Solution:
     def __init__(self):
         self.final_state = None
@@ -348,7 +384,8 @@ def solve(n, indptr, rows, rates, x0, graph, full=True):
             sol.nonterm_data = np.concatenate(dat)
             y = X @ x0[ntv] + np.bincount(scc_of_t, weights=x0[tv], minlength=T)
         else:
-            w = sla.spsolve(Nm, x0[ntv])
+            hv = vertex_heights(n, indptr, rows, graph[3])
+            w = solve_by_levels(Nm, x0[ntv], hv[ntv])
             a_ = -(B @ w)
             y = np.bincount(scc_of_t, weights=x0[tv] + a_, minlength=T)
     xs = np.zeros(n)
@@ -396,7 +433,8 @@ def solve_final(n, indptr, rows, rates, x0, graph):
         tv = reordered[:n_t]
         scc_of_t = np.repeat(np.arange(T), np.diff(offsets[: T + 1]))
         if len(ntv):
-            w = sla.spsolve(K[ntv][:, ntv].tocsc(), x0[ntv])
+            hv = vertex_heights(n, indptr, rows, graph[3])
+            w = solve_by_levels(K[ntv][:, ntv], x0[ntv], hv[ntv])
             a_ = -(K[tv][:, ntv] @ w)
         else:
             a_ = np.zeros(n_t)

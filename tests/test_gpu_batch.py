@@ -223,3 +223,105 @@ def test_autotune_choice_keeps_results(ctx, cfg, seed, B):
         rep = plan.solve(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12, restart=0, graph=graph)
         assert rep["residual"] <= 1e-12
         assert np.abs(d_out.download() - d_ref.download()).sum(axis=1).max() <= 1e-10
+
+
+_N20_ORACLE = {}
+
+
+def _n20_oracle(net, rates, k, x0):
+    """FP64 oracle result of rate set k of the N=20 seed 0 network (cached)."""
+    if k not in _N20_ORACLE:
+        om = read_bnet_text(net.bnet())
+        n = 1 << net.N
+        ip, rows, _ = P.transition_table(om, rates[k])
+        gr = P.transition_graph(n, ip, rows)
+        _N20_ORACLE[k] = P.solve_final(n, ip, rows, rates[k], x0, gr)
+    return _N20_ORACLE[k]
+
+
+@pytest.mark.parametrize("restart", [16, 24, 32])
+def test_headline_kernel_each_restart(ctx, restart):
+    # the bench's schedule: per-set LDS-resident kernel with the FP32 basis in
+    # registers (k_gmres_priv<restart>), N=20 seed 0 (1022 unknowns), FP32 +
+    # FP64 refinement at tol 1e-12, replayed as a HIP graph; every set against
+    # one batched FP64 run, two sets against the oracle
+    from paraexastolog_amd._lib import check, lib
+
+    B = 64
+    net, rates, m, t, g, x0 = setup_net(ctx, "c1", 0, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    d64 = DeviceArray(ctx, (B, n), np.float64)
+    d32 = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    assert plan.info()["unknowns"] == 1022
+    plan.solve(B, dr.ptr, d64.ptr, precision=0, tol=1e-13, restart=32)
+    plan.set_schedule(0, restart)
+    check(lib().esl_profile(ctx.handle, 1), ctx.handle)
+    rep = plan.solve(B, dr.ptr, d32.ptr, precision=1, tol=1e-12, restart=0, graph=1)
+    import ctypes as C
+    name = C.create_string_buffer(64)
+    names = []
+    i = 0
+    while lib().esl_profile_stat(ctx.handle, i, name, 64, None, None, None) == 0:
+        names.append(name.value.decode())
+        i += 1
+    check(lib().esl_profile(ctx.handle, 0), ctx.handle)
+    assert "k_gmres_priv<float>" in names, names
+    assert rep["residual"] <= 1e-12 and rep["fp64_fallback"] == 0
+    x32, x64 = d32.download(), d64.download()
+    l1 = np.abs(x32 - x64).sum(axis=1)
+    assert l1.max() <= 1e-10, (l1.argmax(), l1.max())
+    assert np.abs(x32.sum(axis=1) - 1).max() <= 1e-11
+    for k in (0, 1):
+        ref = _n20_oracle(net, rates, k, x0)
+        assert np.abs(x32[k] - ref).sum() <= 1e-10
+        assert np.abs(x64[k] - ref).sum() <= 1e-12
+
+
+def test_uneven_batch_matches_oracle(ctx):
+    # the bench's uneven batch: 8 networks N = 12..18, 18 (K = 2), 128 rate sets
+    # each, in one esl_plan_solve_many call (FP32 + FP64 refinement, one HIP
+    # graph); three sets of every network against the FP64 oracle, every set
+    # against an FP64 call, mass, non-negativity and global balance K x = 0
+    per = 128
+    specs = []
+    for net, rng in synth.uneven(0):
+        rates = synth.rate_sets(rng, net.N, per)
+        m = model_builder().parse_model(net.bnet())
+        t = transition_table(ctx, m, rates[0])
+        t.construct_table()
+        g = transition_graph(ctx, t)
+        g.reorganize_vertices()
+        fixed, vals = net.fixed_spec()
+        x0 = initial_state(m.nodes, fixed, vals).state
+        dx0 = DeviceArray.from_host(ctx, x0)
+        plan = solve_plan(ctx, t, dx0.ptr)
+        n = 1 << net.N
+        specs.append(dict(net=net, rates=rates, t=t, g=g, x0=x0, dx0=dx0, plan=plan,
+                          dr=DeviceArray.from_host(ctx, rates), d32=DeviceArray(ctx, (per, n), np.float64),
+                          d64=DeviceArray(ctx, (per, n), np.float64)))
+    print("uneven networks:", [(s["net"].N, s["plan"].info()["unknowns"]) for s in specs], flush=True)
+    plans = [s["plan"] for s in specs]
+    reps32 = solve_plan.solve_many(ctx, plans, [per] * 8, [s["dr"].ptr for s in specs],
+                                   [s["d32"].ptr for s in specs], precision=1, tol=1e-12, graph=1)
+    reps64 = solve_plan.solve_many(ctx, plans, [per] * 8, [s["dr"].ptr for s in specs],
+                                   [s["d64"].ptr for s in specs], precision=0, tol=1e-13)
+    for s, r32, r64 in zip(specs, reps32, reps64):
+        net = s["net"]
+        n = 1 << net.N
+        x32, x64 = s["d32"].download(), s["d64"].download()
+        assert r32["residual"] <= 1e-12 and r64["residual"] <= 1e-13, (net.N, r32, r64)
+        assert np.abs(x32 - x64).sum(axis=1).max() <= 1e-10, net.N
+        assert np.abs(x32.sum(axis=1) - 1).max() <= 1e-11, net.N
+        assert x32.min() >= -1e-14, net.N
+        om = read_bnet_text(net.bnet())
+        for k in (0, 1, per - 1):
+            ip, rows, _ = P.transition_table(om, s["rates"][k])
+            gr = P.transition_graph(n, ip, rows)
+            ref = P.solve_final(n, ip, rows, s["rates"][k], s["x0"], gr)
+            assert np.abs(x32[k] - ref).sum() <= 1e-10, (net.N, k, np.abs(x32[k] - ref).sum())
+            assert np.abs(x64[k] - ref).sum() <= 1e-12, (net.N, k, np.abs(x64[k] - ref).sum())
+            K = P.kinetic(n, ip, rows, s["rates"][k])
+            assert np.abs(K @ x32[k]).sum() <= 1e-7, (net.N, k)  # ||K||_1 <= 2 * 10 * N times the L1 bar
