@@ -40,7 +40,7 @@ typedef enum
 	ESL_E_PARSE = 2,	/* malformed .bnet line or formula */
 	ESL_E_ARG = 3,		/* bad argument: unknown node name, negative rate, size mismatch */
 	ESL_E_DEVICE = 4,	/* HIP runtime error, no device, or the library was built without one */
-	ESL_E_NUMERIC = 5,	/* iterative solve did not reach its tolerance */
+	ESL_E_NUMERIC = 5,	/* a solve did not reach its tolerance (after the exact fallback) */
 	ESL_E_OOM = 6,		/* device or host allocation failed */
 	ESL_E_STATE = 7		/* call order violated (e.g. export before build) */
 } esl_status;
@@ -153,6 +153,10 @@ typedef struct
 							* [n_sets][n_t]; host memory for esl_solve, device memory otherwise */
 	double inner_rtol;	 /* precision 1: residual reduction one FP32 cycle aims for before the FP64
 						  * refinement step (default 1e-6; 0 keeps iterating towards tol) */
+	int exact_fallback;	 /* 1 (default): a rate set still above tol after its Krylov solves is solved
+						  * exactly -- FP64 dense LU with partial pivoting of each diagonal block of its
+						  * system (blocks up to 16384 unknowns) plus two refinement steps -- instead of
+						  * failing the call; 0: off; 2: every set is solved that way (a check) */
 } esl_solve_opts;
 
 typedef struct
@@ -168,6 +172,7 @@ typedef struct
 	double ms_total;	 /* device time of the call */
 	double ms_stage[8];	 /* reach, terminal masks, setup, terminal solve, transient solve, final, -, - */
 	int64_t iterations_sum; /* Krylov iterations summed over the rate sets */
+	int exact_solves;	 /* rate sets solved by the exact dense LU (esl_solve_opts.exact_fallback) */
 } esl_solve_report;
 
 void esl_solve_opts_default(esl_solve_opts* o);
@@ -256,6 +261,18 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
  * 1000 * schedule + restart, or -1 before tuning).  No reference counterpart. */
 esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double* d_rates, double* d_final,
 							 const esl_solve_opts* opts, int32_t choice[2], double* best_ms);
+
+/* Per-set outcome of the last solve on p (esl_plan_solve, esl_plan_solve_many,
+ * esl_plan_solve_noalloc, esl_plan_autotune), n_sets as in that call; any output
+ * may be NULL.  residual[i]: relative FP64 residual of set i's systems;
+ * iterations[i]: its Krylov iterations; method[i]: 0 Krylov solve at the asked
+ * precision, 1 finished by FP64 Krylov after its FP32 cycles stalled, 2 exact
+ * dense LU (esl_solve_opts.exact_fallback), -1 above the tolerance (the call
+ * returned ESL_E_NUMERIC; the other sets' results are valid).  The reference
+ * solves one rate set per call and exits on failure (cu_context.cpp:7-32), so
+ * there is no counterpart there. */
+esl_status esl_plan_set_results(const esl_plan* p, int n_sets, double* residual, int32_t* iterations,
+								int32_t* method);
 
 /* Pins the schedule esl_plan_autotune would store (e.g. one found earlier):
  * schedule 0..3 as above with the given restart length, or -1 to forget it. */

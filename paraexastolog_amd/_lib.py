@@ -33,6 +33,7 @@ class SolveOpts(C.Structure):
         ("graph", C.c_int),
         ("warm_pi", C.POINTER(C.c_double)),
         ("inner_rtol", C.c_double),
+        ("exact_fallback", C.c_int),
     ]
 
 
@@ -49,6 +50,7 @@ class SolveReport(C.Structure):
         ("ms_total", C.c_double),
         ("ms_stage", C.c_double * 8),
         ("iterations_sum", C.c_int64),
+        ("exact_solves", C.c_int),
     ]
 
     def as_dict(self):
@@ -61,6 +63,7 @@ class SolveReport(C.Structure):
             "sets_iterated": self.sets_iterated,
             "fp64_fallback": self.fp64_fallback,
             "iterations_sum": self.iterations_sum,
+            "exact_solves": self.exact_solves,
             "ms_total": self.ms_total,
             "ms_stage": list(self.ms_stage)[:5],
         }
@@ -111,6 +114,7 @@ SIGNATURES = [
     ("esl_plan_solve_noalloc", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(SolveReport)]),
     ("esl_plan_gather_terminal", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_plan_set_schedule", C.c_int, [_VP, C.c_int, C.c_int]),
+    ("esl_plan_set_results", C.c_int, [_VP, C.c_int, _P(C.c_double), _P(C.c_int32), _P(C.c_int32)]),
     ("esl_plan_autotune", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(C.c_int32), _P(C.c_double)]),
     ("esl_plan_solve_many", C.c_int, [_VP, C.c_int, _P(_VP), _P(C.c_int), _P(_VP), _P(_VP), _P(SolveOpts),
                                       _P(SolveReport)]),

@@ -269,13 +269,16 @@ class sparse_cs_matrix:
         return len(self.indptr) - 1
 
 
-def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=32, full=0, preconditioner=0, inner_rtol=None):
+def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=32, full=0, preconditioner=0, inner_rtol=None,
+               exact_fallback=None):
     o = SolveOpts()
     lib().esl_solve_opts_default(C.byref(o))
     o.precision, o.tol, o.max_iters, o.restart, o.full = precision, tol, max_iters, restart, full
     o.preconditioner = preconditioner
     if inner_rtol is not None:
         o.inner_rtol = inner_rtol
+    if exact_fallback is not None:
+        o.exact_fallback = exact_fallback
     return o
 
 
@@ -402,6 +405,17 @@ class solve_plan:
         """d_out[n_sets][n_t] = final vectors on the terminal states (warm-start layout)."""
         check(lib().esl_plan_gather_terminal(self.context.handle, self.handle, n_sets, d_final, d_out),
               self.context.handle)
+
+    def set_results(self, n_sets):
+        """Per-set outcome of the last solve: (residual, iterations, method) arrays,
+        method 0 Krylov, 1 FP64 Krylov finish, 2 exact dense LU, -1 missed."""
+        res = np.zeros(n_sets)
+        its = np.zeros(n_sets, dtype=np.int32)
+        meth = np.zeros(n_sets, dtype=np.int32)
+        check(lib().esl_plan_set_results(self.handle, n_sets, res.ctypes.data_as(C.POINTER(C.c_double)),
+                                         its.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         meth.ctypes.data_as(C.POINTER(C.c_int32))))
+        return res, its, meth
 
     def set_schedule(self, schedule, restart=32):
         """Pins a schedule (0 per-set LDS, 1 per-set global, 2 device-wide, 3 per-set LDS with

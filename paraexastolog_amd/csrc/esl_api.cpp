@@ -431,6 +431,7 @@ void esl_solve_opts_default(esl_solve_opts* o)
 	o->graph = 0;
 	o->warm_pi = nullptr;
 	o->inner_rtol = 1e-6;
+	o->exact_fallback = 1;
 }
 
 esl_status esl_solve_batch_device(esl_ctx* ctx, const esl_graph* g, int n_sets, const double* d_rates,
@@ -648,6 +649,16 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
 		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
 		set_device(ctx);
 		esl::plan_gather_terminal(ctx->c, p->p, n_sets, d_final, d_out);
+	});
+}
+
+esl_status esl_plan_set_results(const esl_plan* p, int n_sets, double* residual, int32_t* iterations,
+								int32_t* method)
+{
+	return guarded(nullptr, [&] {
+		require(p != nullptr, ESL_E_ARG, "null plan");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		esl::plan_set_results(p->p, n_sets, residual, iterations, method);
 	});
 }
 

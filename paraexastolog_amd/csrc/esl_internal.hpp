@@ -299,6 +299,7 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 				   int32_t choice[2], double* best_ms);
 void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final, double* d_out);
 void plan_set_schedule(void* p, int path, int restart);
+void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* method);
 // the plan solve whose buffers are slices of the static scratch (no allocation)
 void noalloc_init(ctx_t& c); // once per device, with that device current
 void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,

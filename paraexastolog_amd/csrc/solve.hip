@@ -391,7 +391,8 @@ __global__ __launch_bounds__(sblock) void k_residual(sys_t S, int n_seg, const d
 								[&](int32_t ut, double rate) { acc -= rate * ib[ut] * zb[ut]; });
 		const double bv = rhs[(int64_t)b * S.nU + u];
 		const double res = bv - acc;
-		r[(int64_t)b * S.nU + u] = res;
+		if (r)
+			r[(int64_t)b * S.nU + u] = res;
 		rr = res * res;
 		bb = bv * bv;
 	}
@@ -2592,6 +2593,324 @@ __global__ __launch_bounds__(sblock) void k_warm_z(sys_t S, int n_seg, const dou
 }
 
 // ============================================================================
+// Exact fallback: FP64 dense LU with partial pivoting of one diagonal block of
+// one set's system (the systems are block diagonal: one block per terminal
+// segment, one for the ambiguous states), for sets whose Krylov solve missed
+// the tolerance.  Column-major s x s matrix, right-looking and blocked by
+// lu_nb columns: panel factorisation (one workgroup), row interchanges of the
+// other columns, triangular solve of the block row, and the trailing update
+// A22 -= L21 U12 in 64 x 64 tiles staged through LDS.  Replaces the
+// reference's dense getrf path (linear_system_solve.cu:69-88, 249-325;
+// sparse_utils.cu:268-291), which factors without pivoting.
+
+constexpr int lu_nb = 32;
+constexpr int lu_panel_threads = 1024;
+constexpr int64_t lu_max_block = 16384; // 2 GiB of FP64 matrix
+
+// A = M restricted to block [u0, u0 + s) of set b (x variables: diagonal D'_u,
+// off-diagonal -rate; anchor rows e_u); x = the block of the right-hand side
+__global__ __launch_bounds__(sblock) void k_lu_assemble(sys_t S, nbr_t G, int n_seg, int b, int64_t u0, int s,
+														double* __restrict__ A, const double* __restrict__ rhs,
+														double* __restrict__ x, int32_t* __restrict__ flag)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= s)
+		return;
+	const int64_t u = u0 + i;
+	const int64_t off = (int64_t)b * S.nU;
+	x[i] = rhs[off + u];
+	if (is_anchor(S, n_seg, b, u))
+	{
+		A[(int64_t)i * s + i] = 1.0;
+		return;
+	}
+	const double* __restrict__ r = S.rates + (int64_t)b * S.rate_stride;
+	A[(int64_t)i * s + i] = 1.0 / S.invd[off + u];
+	for (int32_t e = G.ptr[u]; e < G.ptr[u + 1]; ++e)
+	{
+		const int64_t k = (int64_t)G.nu[e] - u0;
+		if (k < 0 || k >= s)
+		{
+			atomicOr(flag, 2); // a coupling leaves the block: the block split is wrong
+			continue;
+		}
+		A[k * s + i] -= r[G.nr[e] & 63];
+	}
+}
+
+// factorises columns [j0, j0 + jb) of rows [j0, s): pivot search, row swap
+// inside the panel, scaling and the rank-1 updates of the panel's own columns
+__global__ __launch_bounds__(lu_panel_threads) void k_lu_panel(double* __restrict__ A, int s, int j0, int jb,
+															   int32_t* __restrict__ ipiv, int32_t* __restrict__ flag)
+{
+	__shared__ double s_val[lu_panel_threads / wave];
+	__shared__ int s_idx[lu_panel_threads / wave];
+	__shared__ double s_prow[lu_nb];
+	__shared__ int s_piv;
+	const int64_t lda = s;
+	const int nw = lu_panel_threads / wave;
+	for (int jj = 0; jj < jb; ++jj)
+	{
+		const int j = j0 + jj;
+		double* __restrict__ colj = A + (int64_t)j * lda;
+		double best = -1.0;
+		int bi = j;
+		for (int i = j + threadIdx.x; i < s; i += lu_panel_threads)
+		{
+			const double v = fabs(colj[i]);
+			if (v > best)
+			{
+				best = v;
+				bi = i;
+			}
+		}
+		for (int d = wave / 2; d > 0; d >>= 1)
+		{
+			const double ov = __shfl_xor(best, d);
+			const int oi = __shfl_xor(bi, d);
+			if (ov > best || (ov == best && oi < bi))
+			{
+				best = ov;
+				bi = oi;
+			}
+		}
+		if (lane_id() == 0)
+		{
+			s_val[threadIdx.x / wave] = best;
+			s_idx[threadIdx.x / wave] = bi;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			double bv = s_val[0];
+			int bx = s_idx[0];
+			for (int w = 1; w < nw; ++w)
+				if (s_val[w] > bv || (s_val[w] == bv && s_idx[w] < bx))
+				{
+					bv = s_val[w];
+					bx = s_idx[w];
+				}
+			if (!(bv > 0.0))
+				atomicOr(flag, 1); // singular (or not finite)
+			s_piv = bx;
+			ipiv[j] = bx;
+		}
+		__syncthreads();
+		const int p = s_piv;
+		if (p != j)
+			for (int k = threadIdx.x; k < jb; k += lu_panel_threads)
+			{
+				double* c = A + (int64_t)(j0 + k) * lda;
+				const double t = c[j];
+				c[j] = c[p];
+				c[p] = t;
+			}
+		__syncthreads();
+		if (threadIdx.x < jb)
+			s_prow[threadIdx.x] = A[(int64_t)(j0 + threadIdx.x) * lda + j];
+		__syncthreads();
+		const double d = s_prow[jj];
+		const double inv = d != 0.0 ? 1.0 / d : 0.0;
+		for (int i = j + 1 + threadIdx.x; i < s; i += lu_panel_threads)
+		{
+			const double l = colj[i] * inv;
+			colj[i] = l;
+			for (int kk = jj + 1; kk < jb; ++kk)
+				A[(int64_t)(j0 + kk) * lda + i] -= l * s_prow[kk];
+		}
+		__syncthreads();
+	}
+}
+
+// the panel's row interchanges applied to every column outside the panel
+__global__ __launch_bounds__(sblock) void k_lu_swap(double* __restrict__ A, int s, int j0, int jb,
+													const int32_t* __restrict__ ipiv)
+{
+	const int c = blockIdx.x * sblock + threadIdx.x;
+	if (c >= s || (c >= j0 && c < j0 + jb))
+		return;
+	double* __restrict__ col = A + (int64_t)c * s;
+	for (int jj = 0; jj < jb; ++jj)
+	{
+		const int j = j0 + jj, p = ipiv[j];
+		if (p != j)
+		{
+			const double t = col[j];
+			col[j] = col[p];
+			col[p] = t;
+		}
+	}
+}
+
+// U12 = L11^{-1} A12: one thread per column right of the panel, the unit
+// lower triangle L11 in LDS
+__global__ __launch_bounds__(sblock) void k_lu_trsm(double* __restrict__ A, int s, int j0, int jb)
+{
+	__shared__ double sL[lu_nb][lu_nb + 1];
+	for (int q = threadIdx.x; q < lu_nb * lu_nb; q += sblock)
+	{
+		const int i = q % lu_nb, k = q / lu_nb;
+		sL[i][k] = (i < jb && k < jb) ? A[(int64_t)(j0 + k) * s + j0 + i] : 0.0;
+	}
+	__syncthreads();
+	const int c = j0 + jb + blockIdx.x * sblock + threadIdx.x;
+	if (c >= s)
+		return;
+	double* __restrict__ col = A + (int64_t)c * s + j0;
+	double x[lu_nb];
+#pragma unroll
+	for (int i = 0; i < lu_nb; ++i)
+	{
+		double v = i < jb ? col[i] : 0.0;
+#pragma unroll
+		for (int k = 0; k < i; ++k)
+			v -= sL[i][k] * x[k];
+		x[i] = v;
+		if (i < jb)
+			col[i] = v;
+	}
+}
+
+// A22 -= L21 U12 on the trailing (s - j0 - jb)^2 block, one 64 x 64 tile per
+// workgroup, 4 x 4 results per thread
+__global__ __launch_bounds__(sblock) void k_lu_gemm(double* __restrict__ A, int s, int j0, int jb)
+{
+	__shared__ double sL[lu_nb][64 + 1];
+	__shared__ double sU[lu_nb][64 + 1];
+	const int base = j0 + jb;
+	const int r0 = base + blockIdx.x * 64, c0 = base + blockIdx.y * 64;
+	for (int q = threadIdx.x; q < lu_nb * 64; q += sblock)
+	{
+		const int r = q % 64, k = q / 64; // rows contiguous in a column of L21
+		sL[k][r] = (k < jb && r0 + r < s) ? A[(int64_t)(j0 + k) * s + r0 + r] : 0.0;
+		const int kk = q % lu_nb, cc = q / lu_nb; // U12 column cc, rows contiguous
+		sU[kk][cc] = (kk < jb && c0 + cc < s) ? A[(int64_t)(c0 + cc) * s + j0 + kk] : 0.0;
+	}
+	__syncthreads();
+	const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
+	double acc[4][4] = {};
+	for (int k = 0; k < jb; ++k)
+	{
+		double a[4], bb[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			a[i] = sL[k][tr + 16 * i];
+			bb[i] = sU[k][tc + 16 * i];
+		}
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+				acc[i][j] += a[i] * bb[j];
+	}
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+		const int c = c0 + tc + 16 * j;
+		if (c >= s)
+			continue;
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			const int r = r0 + tr + 16 * i;
+			if (r < s)
+				A[(int64_t)c * s + r] -= acc[i][j];
+		}
+	}
+}
+
+// x <- A^{-1} x with the factors: the interchanges, then forward substitution
+// with unit L and back substitution with U, x in LDS (s <= lu_max_block)
+__global__ __launch_bounds__(lu_panel_threads) void k_lu_solve(const double* __restrict__ A, int s,
+															   const int32_t* __restrict__ ipiv, double* __restrict__ x)
+{
+	extern __shared__ double sx[];
+	for (int i = threadIdx.x; i < s; i += lu_panel_threads)
+		sx[i] = x[i];
+	__syncthreads();
+	if (threadIdx.x == 0)
+		for (int j = 0; j < s; ++j)
+		{
+			const int p = ipiv[j];
+			if (p != j)
+			{
+				const double t = sx[j];
+				sx[j] = sx[p];
+				sx[p] = t;
+			}
+		}
+	__syncthreads();
+	for (int j = 0; j < s; ++j)
+	{
+		const double xj = sx[j];
+		const double* __restrict__ col = A + (int64_t)j * s;
+		for (int i = j + 1 + threadIdx.x; i < s; i += lu_panel_threads)
+			sx[i] -= col[i] * xj;
+		__syncthreads();
+	}
+	for (int j = s - 1; j >= 0; --j)
+	{
+		const double* __restrict__ col = A + (int64_t)j * s;
+		const double xj = sx[j] / col[j];
+		__syncthreads();
+		if (threadIdx.x == 0)
+			sx[j] = xj;
+		for (int i = threadIdx.x; i < j; i += lu_panel_threads)
+			sx[i] -= col[i] * xj;
+		__syncthreads();
+	}
+	for (int i = threadIdx.x; i < s; i += lu_panel_threads)
+		x[i] = sx[i];
+}
+
+// r = rhs - M x on the block, M applied through the coupling structure (the
+// LU overwrote its copy); used for iterative refinement with the factors
+__global__ __launch_bounds__(sblock) void k_lu_residual(sys_t S, nbr_t G, int n_seg, int b, int64_t u0, int s,
+														const double* __restrict__ x, const double* __restrict__ rhs,
+														double* __restrict__ r)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= s)
+		return;
+	const int64_t u = u0 + i;
+	const int64_t off = (int64_t)b * S.nU;
+	double mx;
+	if (is_anchor(S, n_seg, b, u))
+		mx = x[i];
+	else
+	{
+		const double* __restrict__ rt = S.rates + (int64_t)b * S.rate_stride;
+		mx = x[i] / S.invd[off + u];
+		for (int32_t e = G.ptr[u]; e < G.ptr[u + 1]; ++e)
+		{
+			const int64_t k = (int64_t)G.nu[e] - u0;
+			if (k >= 0 && k < s) // k_lu_assemble flags couplings that leave the block
+				mx -= rt[G.nr[e] & 63] * x[k];
+		}
+	}
+	r[i] = rhs[off + u] - mx;
+}
+
+__global__ __launch_bounds__(sblock) void k_lu_axpy(int s, const double* __restrict__ dx, double* __restrict__ x)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i < s)
+		x[i] += dx[i];
+}
+
+// z = D' x on the block (the solvers' convention outside the kernels)
+__global__ __launch_bounds__(sblock) void k_lu_store(sys_t S, int b, int64_t u0, int s, const double* __restrict__ x,
+													 double* __restrict__ z)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= s)
+		return;
+	const int64_t at = (int64_t)b * S.nU + u0 + i;
+	z[at] = x[i] / S.invd[at];
+}
+
+// ============================================================================
 // host driver
 
 namespace {
@@ -2886,9 +3205,7 @@ struct gmres_t
 			}
 			prev = final_res;
 			if (iters >= max_iters)
-				throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-											   std::to_string(final_res) + " after " + std::to_string(iters) +
-											   " iterations");
+				return false; // the caller checks every set's residual
 			++cycles;
 			k_cast_scale<T><<<grid, sblock, 0, c.stream>>>(nU, r.get(), vscale.get(), V.get());
 			int k = 0;
@@ -2977,6 +3294,7 @@ struct krylov_result_t
 	int fp64_fallback = 0; // sets finished in FP64 after their FP32 cycles stalled
 	int missed = 0;		   // sets above the tolerance
 	int64_t iters_sum = 0; // iterations summed over the sets
+	int exact = 0;		   // sets solved by the exact dense LU
 };
 
 // device scratch of the per-set GMRES kernels (pointers only)
@@ -2986,6 +3304,15 @@ struct pss_view_t
 	int32_t *its = nullptr, *cyc = nullptr, *stl = nullptr;
 	float* Vf = nullptr;  // FP32 basis (precision 1)
 	double* Vd = nullptr; // FP64 basis (precision 0, and the FP64 finish of precision 1)
+};
+
+// per-set outcome of a solve: relative FP64 residual, Krylov iterations and
+// how the set was finished (0 Krylov at the asked precision, 1 FP64 Krylov
+// after stalled FP32 cycles, 2 exact dense LU, -1 above the tolerance)
+struct set_results_t
+{
+	std::vector<double> res;
+	std::vector<int32_t> its, method;
 };
 
 std::string sci(double v)
@@ -3173,27 +3500,47 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 // missed the tolerance (their FP32 cycles stalled) are finished by the FP64
 // kernel from their current iterate; sets that already converged leave it at
 // their first residual check.
-// fp64_view() gives the scratch with an FP64 basis (it may grow it).
+// fp64_view() gives the scratch with an FP64 basis (it may grow it).  Nothing
+// is thrown here: sets still above the tolerance are counted in kr.missed and
+// marked in `out` (method -1) for the caller's fallback.
 template <typename F>
 krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs,
-							   double* z, esl_solve_opts o, const pss_view_t& sc, F&& fp64_view)
+							   double* z, esl_solve_opts o, const pss_view_t& sc, F&& fp64_view,
+							   set_results_t* out = nullptr)
 {
 	ESL_HIP(hipStreamSynchronize(c.stream));
 	const int B = S.B;
-	krylov_result_t kr = collect_sets(c, sc, B, o.tol, o.precision != 1);
-	if (o.precision != 1 || kr.missed == 0)
-		return kr;
-	o.precision = 0;
-	const pss_view_t sd = fp64_view();
-	launch_per_set(c, S, nb, n_seg, rhs, z, o, sd);
-	ESL_HIP(hipStreamSynchronize(c.stream));
-	const krylov_result_t k2 = collect_sets(c, sd, B, o.tol, true);
-	kr.fp64_fallback = kr.missed;
-	kr.missed = 0;
-	kr.resid = k2.resid;
-	kr.iters += k2.iters;
-	kr.cycles += k2.cycles;
-	kr.iters_sum += k2.iters_sum;
+	set_results_t sr;
+	sr.method.assign(B, 0);
+	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its);
+	if (o.precision == 1 && kr.missed > 0)
+	{
+		o.precision = 0;
+		const pss_view_t sd = fp64_view();
+		launch_per_set(c, S, nb, n_seg, rhs, z, o, sd);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		std::vector<double> res2;
+		std::vector<int32_t> its2;
+		const krylov_result_t k2 = collect_sets(c, sd, B, o.tol, false, &res2, &its2);
+		for (int b = 0; b < B; ++b)
+		{
+			if (!(sr.res[b] <= o.tol))
+				sr.method[b] = 1;
+			sr.res[b] = res2[b];
+			sr.its[b] += its2[b];
+		}
+		kr.fp64_fallback = kr.missed;
+		kr.missed = k2.missed;
+		kr.resid = k2.resid;
+		kr.iters += k2.iters;
+		kr.cycles += k2.cycles;
+		kr.iters_sum += k2.iters_sum;
+	}
+	for (int b = 0; b < B; ++b)
+		if (!(sr.res[b] <= o.tol))
+			sr.method[b] = -1;
+	if (out)
+		*out = std::move(sr);
 	return kr;
 }
 
@@ -3243,6 +3590,7 @@ struct plan_t
 	} tune;
 	bool allow_lds = true; // the schedule of the run being enqueued
 	bool allow_reg = true;
+	set_results_t results; // per-set outcome of the last run (esl_plan_set_results)
 	double ms[3] = { 0, 0, 0 };
 	// captured numeric sequence and the arguments it was captured with
 	struct gkey_t
@@ -3287,8 +3635,10 @@ struct plan_t
 };
 
 
+// throw_on_miss = false leaves sets above the tolerance to the caller (kr.missed)
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
-							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr, bool device_wide = false)
+							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr, bool device_wide = false,
+							 bool throw_on_miss = true, std::vector<double>* set_res = nullptr)
 {
 	krylov_result_t kr;
 	const int64_t nU = S.nU;
@@ -3314,6 +3664,9 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 			sc.reserve(B, nU, m, 0);
 			return sc.view();
 		});
+		if (kr.missed && throw_on_miss)
+			throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " + sci(kr.resid) +
+										   " after " + std::to_string(kr.iters) + " iterations");
 	}
 	else
 	{
@@ -3330,6 +3683,8 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		{
 			gmres_t<float> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
 			done = gm.solve(rhs_p, z_p, tol, max_iters, opts.inner_rtol);
+			if (set_res)
+				*set_res = gm.h_res; // the residuals of its last check
 			kr.resid = gm.final_res;
 			kr.iters = gm.iters;
 			kr.cycles = gm.cycles;
@@ -3340,10 +3695,20 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 		{
 			// FP64 from the current iterate (also what an FP32 solve that stalled falls back to)
 			gmres_t<double> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			gm.solve(rhs_p, z_p, tol, std::max(max_iters - kr.iters, 1));
+			const bool done64 = gm.solve(rhs_p, z_p, tol, std::max(max_iters - kr.iters, 1));
+			if (set_res)
+				*set_res = gm.h_res;
 			kr.resid = gm.final_res;
 			kr.iters += gm.iters;
 			kr.cycles += gm.cycles;
+			if (!done64)
+			{
+				if (throw_on_miss)
+					throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+												   sci(kr.resid) + " after " + std::to_string(kr.iters) +
+												   " iterations");
+				kr.missed = B; // an upper bound: the caller reads every set's residual
+			}
 		}
 		kr.sets_iterated = kr.cycles > 0 ? B : 0;
 		kr.iters_sum = (int64_t)kr.iters * B; // the sets iterate together
@@ -3747,6 +4112,188 @@ void plan_reserve_(plan_t& P, int B, int m, int precision)
 	P.use_own_bufs();
 }
 
+// relative FP64 residual ||rhs - A_p z||_2 / ||rhs||_2 of every set, from the
+// mask-based operator (independent of the solvers' coupling structure)
+std::vector<double> set_residuals(ctx_t& c, const sys_t& S, int n_seg, const double* rhs, const double* z)
+{
+	const int B = S.B;
+	std::vector<double> out(B, 0.0);
+	if (S.nU == 0)
+		return out;
+	const int nblk = ceil_div(S.nU, sblock);
+	dbuf<double> part, rr;
+	part.ensure((size_t)B * nblk * 2);
+	rr.ensure((size_t)B * 2);
+	k_residual<<<dim3(nblk, B), sblock, 0, c.stream>>>(S, std::max(n_seg, 1), rhs, z, nullptr, part.get());
+	k_reduce_parts<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nblk, 2, rr.get(), 2, 0);
+	ESL_HIP(hipGetLastError());
+	std::vector<double> h;
+	read_host(c, h, rr.get(), (size_t)B * 2);
+	for (int b = 0; b < B; ++b)
+	{
+		const double r2 = h[2 * b], b2 = h[2 * b + 1];
+		out[b] = b2 > 0 ? std::sqrt(r2 / b2) : (r2 > 0 ? INFINITY : 0.0);
+	}
+	return out;
+}
+
+// the diagonal blocks (first unknown, size) of every set's system: one per
+// terminal segment, one for the ambiguous states
+std::vector<std::pair<int64_t, int64_t>> plan_blocks(const plan_t& P)
+{
+	std::vector<std::pair<int64_t, int64_t>> bl;
+	for (int k = 0; k < P.n_seg; ++k)
+		bl.push_back({ P.seg_off[k], P.seg_off[k + 1] - P.seg_off[k] });
+	if (P.nA > 0)
+		bl.push_back({ P.nUt, P.nA });
+	return bl;
+}
+
+// Exact FP64 solve of the systems of `sets` (indices into the batch) by dense
+// LU with partial pivoting, block by block, with two steps of iterative
+// refinement; z of those sets receives D' x.  Returns false, changing nothing,
+// when a block exceeds lu_max_block unknowns.
+bool exact_solve_sets(ctx_t& c, plan_t& P, int B, const double* d_rates, const std::vector<int>& sets)
+{
+	const auto blocks = plan_blocks(P);
+	int64_t smax = 0;
+	for (auto& bl : blocks)
+		smax = std::max(smax, bl.second);
+	if (smax == 0 || sets.empty())
+		return true;
+	if (smax > lu_max_block)
+		return false;
+	dbuf<double> A, x, r;
+	dbuf<int32_t> ipiv, flag;
+	A.ensure((size_t)smax * smax);
+	x.ensure(smax);
+	r.ensure(smax);
+	ipiv.ensure(smax);
+	flag.ensure(1);
+	k_zero<int32_t><<<1, sblock, 0, c.stream>>>(1, flag.get());
+	if (smax * (int64_t)sizeof(double) > 48 * 1024)
+		ESL_HIP(hipFuncSetAttribute((const void*)k_lu_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+									(int)(smax * sizeof(double))));
+	const sys_t S = P.sys(B, d_rates);
+	const nbr_t G = P.nbr.view();
+	const int nseg1 = std::max(P.n_seg, 1);
+	for (int b : sets)
+		for (auto& bl : blocks)
+		{
+			const int64_t u0 = bl.first;
+			const int sz = (int)bl.second;
+			const int gs = ceil_div(sz, sblock);
+			const size_t lds = (size_t)sz * sizeof(double);
+			k_zero<double><<<ceil_div((int64_t)sz * sz, sblock), sblock, 0, c.stream>>>((int64_t)sz * sz, A.get());
+			k_lu_assemble<<<gs, sblock, 0, c.stream>>>(S, G, nseg1, b, u0, sz, A.get(), P.rb.rhs, x.get(),
+													   flag.get());
+			for (int j0 = 0; j0 < sz; j0 += lu_nb)
+			{
+				const int jb = std::min(lu_nb, sz - j0);
+				k_lu_panel<<<1, lu_panel_threads, 0, c.stream>>>(A.get(), sz, j0, jb, ipiv.get(), flag.get());
+				k_lu_swap<<<gs, sblock, 0, c.stream>>>(A.get(), sz, j0, jb, ipiv.get());
+				const int rest = sz - j0 - jb;
+				if (rest > 0)
+				{
+					k_lu_trsm<<<ceil_div(rest, sblock), sblock, 0, c.stream>>>(A.get(), sz, j0, jb);
+					const int t = ceil_div(rest, 64);
+					k_lu_gemm<<<dim3(t, t), sblock, 0, c.stream>>>(A.get(), sz, j0, jb);
+				}
+			}
+			k_lu_solve<<<1, lu_panel_threads, lds, c.stream>>>(A.get(), sz, ipiv.get(), x.get());
+			for (int it = 0; it < 2; ++it) // iterative refinement with the factors
+			{
+				k_lu_residual<<<gs, sblock, 0, c.stream>>>(S, G, nseg1, b, u0, sz, x.get(), P.rb.rhs, r.get());
+				k_lu_solve<<<1, lu_panel_threads, lds, c.stream>>>(A.get(), sz, ipiv.get(), r.get());
+				k_lu_axpy<<<gs, sblock, 0, c.stream>>>(sz, r.get(), x.get());
+			}
+			k_lu_store<<<gs, sblock, 0, c.stream>>>(S, b, u0, sz, x.get(), P.rb.z);
+			ESL_HIP(hipGetLastError());
+		}
+	int32_t h_flag = 0;
+	ESL_HIP(hipMemcpyAsync(&h_flag, flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	if (h_flag & 2)
+		throw error(ESL_E_DEVICE, "exact solve: a coupling crosses the diagonal blocks of the system");
+	if (h_flag & 1)
+		throw error(ESL_E_NUMERIC, "exact solve: a system matrix is singular");
+	return true;
+}
+
+// After the Krylov solves of a run: sets above the tolerance (every set when
+// exact_fallback is 2) are solved by the exact dense LU when allow_lu and their
+// blocks fit, the final vectors are rebuilt when a set changed (or `rebuild`),
+// and the per-set outcome is stored in the plan.  A set still above the
+// tolerance fails the call with ESL_E_NUMERIC, after every set's final vector
+// has been written.
+void plan_finish_sets(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,
+					  krylov_result_t& kr, set_results_t sr, bool rebuild, bool allow_lu)
+{
+	std::vector<int> todo;
+	if (P.nU > 0 && o.exact_fallback >= 1)
+		for (int b = 0; b < B; ++b)
+			if (o.exact_fallback == 2 || !(sr.res[b] <= o.tol))
+				todo.push_back(b);
+	bool fits = true;
+	if (!todo.empty() && allow_lu)
+	{
+		fits = exact_solve_sets(c, P, B, d_rates, todo);
+		if (fits)
+		{
+			const auto res = set_residuals(c, P.sys(B, d_rates), P.n_seg, P.rb.rhs, P.rb.z);
+			for (int b : todo)
+			{
+				sr.res[b] = res[b];
+				sr.method[b] = 2;
+			}
+			kr.exact = (int)todo.size();
+			rebuild = true;
+		}
+	}
+	if (rebuild)
+	{
+		plan_enqueue_post(c, P, B, d_rates, d_final);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+	}
+	int missed = 0;
+	double worst = 0.0;
+	for (int b = 0; b < B; ++b)
+	{
+		worst = std::max(worst, std::isnan(sr.res[b]) ? 1e300 : sr.res[b]);
+		if (!(sr.res[b] <= o.tol))
+		{
+			sr.method[b] = -1;
+			++missed;
+		}
+	}
+	kr.resid = worst;
+	kr.missed = missed;
+	P.results = std::move(sr);
+	if (missed)
+	{
+		std::string why;
+		if (o.exact_fallback < 1)
+			why = " (exact fallback off)";
+		else if (!allow_lu)
+			why = " (no exact fallback in this entry point)";
+		else if (!fits)
+			why = " (a system block exceeds the exact solver's " + std::to_string(lu_max_block) + " unknowns)";
+		throw error(ESL_E_NUMERIC, std::to_string(missed) + " of " + std::to_string(B) +
+									   " rate sets missed the tolerance " + sci(o.tol) + ": worst relative residual " +
+									   sci(worst) + why);
+	}
+}
+
+// per-set outcome when there is nothing to solve
+set_results_t trivial_results(int B)
+{
+	set_results_t sr;
+	sr.res.assign(B, 0.0);
+	sr.its.assign(B, 0);
+	sr.method.assign(B, 0);
+	return sr;
+}
+
 void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
 			  esl_solve_report& rep)
 {
@@ -3820,24 +4367,31 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		{
 			esl_solve_opts of = o;
 			of.restart = m;
+			set_results_t sr;
 			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, of, P.rb.ps, [&] {
 				P.scr.reserve(B, P.nU, m, 0);
 				P.use_own_bufs();
 				return P.rb.ps;
-			});
-			if (kr.fp64_fallback) // the final vectors are rebuilt from the FP64 iterates
-				plan_enqueue_post(c, P, B, d_rates, d_final);
-			ESL_HIP(hipStreamSynchronize(c.stream));
+			}, &sr);
+			// the final vectors are rebuilt when FP64 iterates replaced FP32 ones
+			plan_finish_sets(c, P, B, d_rates, d_final, of, kr, std::move(sr), kr.fp64_fallback > 0, true);
 		}
+		else
+			P.results = trivial_results(B);
 	}
 	else
 	{
 		// one large system: device-wide GMRES (host-driven restarts)
 		plan_reserve(P, B, 2, 0);
 		plan_enqueue_setup(c, P, B, d_rates, o);
-		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rb.rhs, P.rb.z, o, &P.nbr, true);
+		set_results_t sr;
+		kr = krylov_solve(c, P.sys(B, d_rates), P.n_seg, P.rb.rhs, P.rb.z, o, &P.nbr, true, false, &sr.res);
 		plan_enqueue_post(c, P, B, d_rates, d_final);
-		ESL_HIP(hipStreamSynchronize(c.stream));
+		if ((int)sr.res.size() != B)
+			sr.res = set_residuals(c, P.sys(B, d_rates), P.n_seg, P.rb.rhs, P.rb.z);
+		sr.its.assign(B, kr.iters);
+		sr.method.assign(B, kr.fp64_fallback ? 1 : 0);
+		plan_finish_sets(c, P, B, d_rates, d_final, o, kr, std::move(sr), false, true);
 	}
 	rep.residual = kr.resid;
 	rep.iterations = kr.iters;
@@ -3845,6 +4399,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	rep.sets_iterated = kr.sets_iterated;
 	rep.fp64_fallback = kr.fp64_fallback;
 	rep.iterations_sum = kr.iters_sum;
+	rep.exact_solves = kr.exact;
 	rep.ms_stage[3] = tm_.lap();
 	rep.ms_total = rep.ms_stage[3];
 }
@@ -4003,10 +4558,14 @@ void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double*
 		if (P.nU > 0)
 		{
 			const pss_view_t ps = P.rb.ps;
-			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, o, ps, [&] { return ps; });
-			if (kr.fp64_fallback) // the final vectors are rebuilt from the FP64 iterates
-				plan_enqueue_post(c, P, B, d_rates, d_final);
+			set_results_t sr;
+			kr = finish_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, o, ps, [&] { return ps; },
+								&sr);
+			// no exact fallback here (its dense matrix would need an allocation)
+			plan_finish_sets(c, P, B, d_rates, d_final, o, kr, std::move(sr), kr.fp64_fallback > 0, false);
 		}
+		else
+			P.results = trivial_results(B);
 		ESL_HIP(hipEventRecord(c.ev[1], c.stream));
 		ESL_HIP(hipEventSynchronize(c.ev[1]));
 		float ms = 0;
@@ -4209,14 +4768,15 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		if (P.nU > 0)
 		{
 			const esl_solve_opts& of = po[i];
-			const krylov_result_t kr =
+			set_results_t sr;
+			krylov_result_t kr =
 				finish_per_set(c, P.sys(B[i], d_rates[i]), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, of, P.rb.ps, [&] {
 					P.scr.reserve(B[i], P.nU, of.restart, 0);
 					P.use_own_bufs();
 					return P.rb.ps;
-				});
-			if (kr.fp64_fallback)
-				plan_enqueue_post(c, P, B[i], d_rates[i], d_final[i]);
+				}, &sr);
+			plan_finish_sets(c, P, B[i], d_rates[i], d_final[i], of, kr, std::move(sr), kr.fp64_fallback > 0, true);
+			rep.exact_solves = kr.exact;
 			rep.residual = kr.resid;
 			rep.iterations = kr.iters;
 			rep.refinements = kr.cycles;
@@ -4267,6 +4827,7 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 	plan_t& P = *static_cast<plan_t*>(p);
 	esl_solve_opts o = o_in;
 	o.restart = 0;
+	o.exact_fallback = 0; // a schedule that misses the tolerance is skipped, not rescued
 	struct cand_t
 	{
 		int path, m;
@@ -4337,6 +4898,25 @@ void plan_set_schedule(void* p, int path, int restart)
 		P.tune = plan_t::tune_t{};
 	else
 		P.tune = plan_t::tune_t{ true, path, std::min(std::max(restart, 2), pm_max) };
+}
+
+void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* method)
+{
+	const plan_t& P = *static_cast<const plan_t*>(p);
+	if (P.results.res.empty())
+		throw error(ESL_E_STATE, "no solve has run on this plan");
+	if ((size_t)B != P.results.res.size())
+		throw error(ESL_E_ARG, "n_sets (" + std::to_string(B) + ") differs from the last solve's (" +
+								   std::to_string(P.results.res.size()) + ")");
+	for (int b = 0; b < B; ++b)
+	{
+		if (res)
+			res[b] = P.results.res[b];
+		if (its)
+			its[b] = P.results.its[b];
+		if (method)
+			method[b] = P.results.method[b];
+	}
 }
 
 void plan_info(const void* p, int64_t info[8])

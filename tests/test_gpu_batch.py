@@ -325,3 +325,84 @@ def test_uneven_batch_matches_oracle(ctx):
             assert np.abs(x64[k] - ref).sum() <= 1e-12, (net.N, k, np.abs(x64[k] - ref).sum())
             K = P.kinetic(n, ip, rows, s["rates"][k])
             assert np.abs(K @ x32[k]).sum() <= 1e-7, (net.N, k)  # ||K||_1 <= 2 * 10 * N times the L1 bar
+
+
+def _oracle_final(net, rates, k, x0):
+    om = read_bnet_text(net.bnet())
+    n = 1 << net.N
+    ip, rows, _ = P.transition_table(om, rates[k])
+    gr = P.transition_graph(n, ip, rows)
+    return P.solve_final(n, ip, rows, rates[k], x0, gr)
+
+
+@pytest.mark.parametrize("cfg,seed,B", [("c0", 10, 5), ("c1", 0, 3), ("c2", 11, 2)])
+def test_exact_lu_matches_oracle(ctx, cfg, seed, B):
+    # exact_fallback 2: every set solved by the dense LU (+ refinement) of each
+    # diagonal block -- c0 seed 10 has a terminal segment and ambiguous states,
+    # c1 seed 0 one terminal SCC of 1022 states, c2 seed 11 one of 2048
+    net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, exact_fallback=2)
+    assert rep["exact_solves"] == B
+    res, its, meth = plan.set_results(B)
+    assert (meth == 2).all() and res.max() <= 1e-14, (meth, res)
+    xb = dout.download()
+    for k in range(B):
+        ref = _oracle_final(net, rates, k, x0)
+        assert np.abs(xb[k] - ref).sum() <= 1e-12, (k, np.abs(xb[k] - ref).sum())
+
+
+@pytest.mark.parametrize("schedule", [None, 2])
+def test_missed_tolerance_falls_back_to_exact_lu(ctx, schedule):
+    # one Krylov iteration cannot reach 1e-12: every set goes to the exact LU
+    # (per-set kernels, and the device-wide solver when pinned); without the
+    # fallback the call fails with ESL_E_NUMERIC and every set is marked
+    from paraexastolog_amd import EslError
+
+    B = 4
+    net, rates, m, t, g, x0 = setup_net(ctx, "c0", 10, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    restart = 32
+    if schedule is not None:
+        plan.set_schedule(schedule, 32)
+        restart = 0
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, max_iters=1, restart=restart)
+    assert rep["exact_solves"] == B and rep["residual"] <= 1e-12
+    res, its, meth = plan.set_results(B)
+    assert (meth == 2).all()
+    xb = dout.download()
+    for k in range(B):
+        assert np.abs(xb[k] - _oracle_final(net, rates, k, x0)).sum() <= 1e-12
+    with pytest.raises(EslError) as e:
+        plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, max_iters=1, restart=restart, exact_fallback=0)
+    assert "NUMERIC" in str(e.value) and "exact fallback off" in str(e.value)
+    res, its, meth = plan.set_results(B)
+    assert (meth == -1).all() and (res > 1e-12).all()
+
+
+def test_exact_fallback_refuses_blocks_beyond_its_limit(ctx):
+    # c2 seed 37: 32,768 ambiguous states in one block, beyond the exact
+    # solver's 16384: a forced Krylov failure is reported, not rescued
+    from paraexastolog_amd import EslError
+
+    B = 2
+    net, rates, m, t, g, x0 = setup_net(ctx, "c2", 37, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    assert plan.info()["ambiguous"] > 16384
+    with pytest.raises(EslError) as e:
+        plan.solve(B, dr.ptr, dout.ptr, precision=0, tol=1e-13, max_iters=1)
+    assert "exceeds" in str(e.value)
+    res, its, meth = plan.set_results(B)
+    assert (meth == -1).all()

@@ -53,3 +53,23 @@ def test_noalloc_rejects_batches_beyond_the_scratch(ctx):
     with pytest.raises(EslError) as e:
         plan.solve_noalloc(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12)
     assert "scratch" in str(e.value) and "ARG" in str(e.value)
+
+
+def test_noalloc_reports_missed_sets_without_exact_fallback(ctx):
+    # the no-allocation solve has no exact fallback (its dense matrix would need
+    # an allocation): a forced Krylov failure is ESL_E_NUMERIC with every set marked
+    B = 3
+    net, rates, m, t, g, x0 = setup_net(ctx, "c0", 10, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    dr = DeviceArray.from_host(ctx, rates)
+    d_out = DeviceArray(ctx, (B, n), np.float64)
+    with pytest.raises(EslError) as e:
+        plan.solve_noalloc(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12, max_iters=1)
+    assert "NUMERIC" in str(e.value) and "no exact fallback" in str(e.value)
+    res, its, meth = plan.set_results(B)
+    assert (meth == -1).all()
+    rep = plan.solve_noalloc(B, dr.ptr, d_out.ptr, precision=1, tol=1e-12)
+    res, its, meth = plan.set_results(B)
+    assert (meth >= 0).all() and res.max() <= 1e-12 and rep["exact_solves"] == 0
