@@ -227,35 +227,32 @@ __global__ __launch_bounds__(sblock) void k_fill_term_unknowns(int64_t count, co
 	pos[s] = (int32_t)u;
 }
 
-// x0 mass that ends in terminal SCC j without any linear solve:
-// terminal states of j and unambiguous transient states bound for j.
-__global__ __launch_bounds__(sblock) void k_y0(int64_t n, const int32_t* __restrict__ rank, int64_t T,
-											   const int32_t* __restrict__ jmap, const uint8_t* __restrict__ reach,
-											   const uint64_t* __restrict__ tm, const int32_t* __restrict__ amb,
-											   const double* __restrict__ x0, int Tr, double* __restrict__ part)
+// Target of the x0 mass of a state that needs no linear solve: the terminal
+// SCC it lies in, or the one an unambiguous transient state is bound for
+// (compact index j < Tr); -1 for states without mass, unreachable ones and
+// ambiguous ones.  The sums y0[j] are taken on the host in state order, so
+// they do not depend on the launch.
+__global__ __launch_bounds__(sblock) void k_target_of(int64_t n, const int32_t* __restrict__ rank, int64_t T,
+													  const int32_t* __restrict__ jmap,
+													  const uint8_t* __restrict__ reach,
+													  const uint64_t* __restrict__ tm, const int32_t* __restrict__ amb,
+													  const double* __restrict__ x0, int Tr, int32_t* __restrict__ tgt)
 {
-	// part: [gridDim.x][Tr]; one slot per block and target
-	extern __shared__ double s_acc[];
-	for (int j = threadIdx.x; j < Tr; j += sblock)
-		s_acc[j] = 0.0;
-	__syncthreads();
 	const int64_t v = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	if (v < n && reach[v] && !amb[v] && x0[v] != 0.0)
+	if (v >= n)
+		return;
+	int j = -1;
+	if (reach[v] && !amb[v] && x0[v] != 0.0)
 	{
 		const int32_t k = rank[v];
-		int j = -1;
 		if (k < T)
 			j = jmap[k];
 		else if (Tr == 1)
 			j = 0; // one reachable terminal SCC: every reachable state ends there
 		else if (tm[v])
 			j = __ffsll((unsigned long long)tm[v]) - 1;
-		if (j >= 0)
-			atomicAdd(&s_acc[j], x0[v]); // LDS atomics: order within a block only
 	}
-	__syncthreads();
-	for (int j = threadIdx.x; j < Tr; j += sblock)
-		part[(int64_t)blockIdx.x * Tr + j] = s_acc[j];
+	tgt[v] = j;
 }
 
 // ============================================================================
@@ -872,6 +869,93 @@ __global__ __launch_bounds__(sblock) void k_sys_setup(sys_t S, int n_seg, const 
 // ============================================================================
 // assembly of the final vector
 
+// Exit edges of the ambiguous set: edges t -> s from an ambiguous unknown u
+// (state t) to a state s outside the systems' ambiguous part, with the
+// terminal SCC j their mass goes to.  Rate independent, built once per plan;
+// listed per u in bit order, then sorted by j on the host (stable), so the
+// per-set sums run in a fixed order.
+__device__ __forceinline__ int exit_target(const sys_t& S, const int32_t* __restrict__ rank, int64_t T,
+										   const int32_t* __restrict__ jmap, const uint64_t* __restrict__ tm,
+										   uint32_t s)
+{
+	const int32_t us = S.pos[s];
+	if (us >= (int32_t)S.nUt)
+		return -1; // stays ambiguous
+	const int32_t k = rank[s];
+	if (k < T)
+		return jmap[k];
+	return __ffsll((unsigned long long)tm[s]) - 1;
+}
+
+__global__ __launch_bounds__(sblock) void k_exit_count(sys_t S, const int32_t* __restrict__ rank, int64_t T,
+													   const int32_t* __restrict__ jmap,
+													   const uint64_t* __restrict__ tm, int32_t* __restrict__ cnt)
+{
+	const int64_t a = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (a >= S.nU - S.nUt)
+		return;
+	const uint32_t t = (uint32_t)S.ulist[S.nUt + a];
+	uint32_t m = S.mask[t];
+	int c = 0;
+	while (m)
+	{
+		const int i = __ffs(m) - 1;
+		m &= m - 1;
+		c += exit_target(S, rank, T, jmap, tm, t ^ (1u << i)) >= 0 ? 1 : 0;
+	}
+	cnt[a] = c;
+}
+
+__global__ __launch_bounds__(sblock) void k_exit_fill(sys_t S, const int32_t* __restrict__ rank, int64_t T,
+													  const int32_t* __restrict__ jmap,
+													  const uint64_t* __restrict__ tm,
+													  const int32_t* __restrict__ ptr, int32_t* __restrict__ ej,
+													  int32_t* __restrict__ eu, uint8_t* __restrict__ er)
+{
+	const int64_t a = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (a >= S.nU - S.nUt)
+		return;
+	const int64_t u = S.nUt + a;
+	const uint32_t t = (uint32_t)S.ulist[u];
+	uint32_t m = S.mask[t];
+	int32_t at = ptr[a];
+	while (m)
+	{
+		const int i = __ffs(m) - 1;
+		m &= m - 1;
+		const int j = exit_target(S, rank, T, jmap, tm, t ^ (1u << i));
+		if (j < 0)
+			continue;
+		ej[at] = j;
+		eu[at] = (int32_t)u;
+		er[at] = (uint8_t)(2 * i + ((t >> i) & 1u)); // up flip: rates[2i], down: rates[2i + 1]
+		++at;
+	}
+}
+
+// y[b][j] = flow from the ambiguous set into terminal SCC j for set b: the sum
+// over j's exit edges of v_t * rate, v = z * invd (expected time in t).  One
+// workgroup per (j, set), fixed summation order.
+__global__ __launch_bounds__(sblock) void k_exit_sum(sys_t S, const int32_t* __restrict__ ex_off,
+													 const int32_t* __restrict__ ex_u, const uint8_t* __restrict__ ex_r,
+													 const double* __restrict__ z, int Tr, double* __restrict__ y)
+{
+	__shared__ double s_red[sblock / wave + 1];
+	const int j = blockIdx.x;
+	const int b = blockIdx.y;
+	const double* __restrict__ r = S.rates + (int64_t)b * S.rate_stride;
+	const int64_t off = (int64_t)b * S.nU;
+	double acc = 0.0;
+	for (int32_t e = ex_off[j] + threadIdx.x; e < ex_off[j + 1]; e += sblock)
+	{
+		const int32_t u = ex_u[e];
+		acc += z[off + u] * S.invd[off + u] * r[ex_r[e]];
+	}
+	acc = block_sum<sblock>(acc, s_red);
+	if (threadIdx.x == 0)
+		y[(int64_t)b * Tr + j] = acc;
+}
+
 // segment sums of pi (= z * invd) over each terminal segment, per set
 __global__ __launch_bounds__(sblock) void k_seg_sum(int64_t nU, const int32_t* __restrict__ seg_off,
 													const double* __restrict__ z, const double* __restrict__ invd,
@@ -886,47 +970,6 @@ __global__ __launch_bounds__(sblock) void k_seg_sum(int64_t nU, const int32_t* _
 	acc = block_sum<sblock>(acc, s_red);
 	if (threadIdx.x == 0)
 		sums[(int64_t)b * n_seg + k] = acc;
-}
-
-// flow out of the ambiguous set towards each terminal target, per block
-__global__ __launch_bounds__(sblock) void k_exit_flow(sys_t S, const int32_t* __restrict__ rank, int64_t T,
-													  const int32_t* __restrict__ jmap,
-													  const uint64_t* __restrict__ tm, const double* __restrict__ z,
-													  int Tr, double* __restrict__ part)
-{
-	extern __shared__ double s_acc[];
-	for (int j = threadIdx.x; j < Tr; j += sblock)
-		s_acc[j] = 0.0;
-	__syncthreads();
-	const int64_t u = S.nUt + (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u < S.nU)
-	{
-		const double* r = S.rates + (int64_t)b * S.rate_stride;
-		const uint32_t t = (uint32_t)S.ulist[u];
-		const double vt = z[(int64_t)b * S.nU + u] * S.invd[(int64_t)b * S.nU + u];
-		uint32_t m = S.mask[t];
-		while (m)
-		{
-			const int i = __ffs(m) - 1;
-			m &= m - 1;
-			const uint32_t s = t ^ (1u << i);
-			const int32_t us = S.pos[s];
-			if (us >= (int32_t)S.nUt)
-				continue; // stays ambiguous
-			const int32_t k = rank[s];
-			int j;
-			if (k < T)
-				j = jmap[k];
-			else
-				j = __ffsll((unsigned long long)tm[s]) - 1;
-			if (j >= 0)
-				atomicAdd(&s_acc[j], vt * rate_of(r, i, ((t >> i) & 1u) == 0));
-		}
-	}
-	__syncthreads();
-	for (int j = threadIdx.x; j < Tr; j += sblock)
-		part[((int64_t)b * gridDim.x + blockIdx.x) * Tr + j] = s_acc[j];
 }
 
 // final[b][s] for every terminal state s of a reached terminal SCC
@@ -3557,9 +3600,15 @@ struct plan_t
 	bool fallback = false;
 	int64_t nU = 0, nUt = 0, nA = 0;
 	dbuf<double> y0;
+	// exit edges of the ambiguous set sorted by target (k_exit_sum): segment
+	// offsets (host and device), unknown and rate index of each edge
+	std::vector<int32_t> ex_off;
+	dbuf<int32_t> d_ex_off, ex_u;
+	dbuf<uint8_t> ex_r;
+	int64_t n_exit = 0;
 	nbr_store_t nbr;
 	// per-run buffers (grown to the largest batch seen)
-	dbuf<double> p0, p1, invd, rhs, z, y, segsum, fpart;
+	dbuf<double> p0, p1, invd, rhs, z, y, segsum;
 	dbuf<int32_t> anchor;
 	per_set_scratch_t scr;
 	// the buffers the run being enqueued uses: the ones above (plan_reserve), or
@@ -3567,13 +3616,13 @@ struct plan_t
 	struct run_bufs_t
 	{
 		double *p0 = nullptr, *p1 = nullptr, *invd = nullptr, *rhs = nullptr, *z = nullptr, *y = nullptr,
-			   *segsum = nullptr, *fpart = nullptr;
+			   *segsum = nullptr;
 		int32_t* anchor = nullptr;
 		pss_view_t ps;
 	} rb;
 	void use_own_bufs()
 	{
-		rb = run_bufs_t{ p0.get(), p1.get(), invd.get(), rhs.get(), z.get(), y.get(), segsum.get(), fpart.get(),
+		rb = run_bufs_t{ p0.get(), p1.get(), invd.get(), rhs.get(), z.get(), y.get(), segsum.get(),
 						 anchor.get(), scr.view() };
 	}
 	int anchor_sweeps = 32;
@@ -3779,8 +3828,6 @@ void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 			P.seg_src.push_back(g.h_offsets[k]);
 		}
 	}
-	if (P.Tr > 4096)
-		throw error(ESL_E_ARG, "more than 4096 terminal SCCs are reachable from the initial states");
 	P.fallback = P.Tr > max_rhs_terms;
 	P.d_jmap.ensure(std::max<int64_t>(T, 1));
 	P.d_segmap.ensure(std::max<int64_t>(T, 1));
@@ -3852,18 +3899,77 @@ void plan_build(ctx_t& c, graph_t& g, const double* d_x0, plan_t& P)
 		k_fill_unknowns<<<nb, sblock, 0, c.stream>>>(n, P.amb.get(), fpos.get(), P.nUt, P.ulist.get(), P.ucls.get(),
 													 P.pos.get());
 
-	// ---- x0 mass needing no solve
+	// ---- x0 mass needing no solve, summed on the host in state order
 	P.y0.ensure(std::max(P.Tr, 1));
 	if (P.Tr)
 	{
-		dbuf<double> y0part;
-		y0part.ensure((size_t)nb * P.Tr);
-		k_y0<<<nb, sblock, P.Tr * sizeof(double), c.stream>>>(n, g.rank.get(), T, P.d_jmap.get(), P.reach.get(),
-															   P.tmask.get(), P.amb.get(), P.x0.get(), P.Tr,
-															   y0part.get());
-		k_reduce_parts<<<dim3(P.Tr, 1), wave, 0, c.stream>>>(y0part.get(), nb, P.Tr, P.y0.get(), P.Tr, 0);
+		dbuf<int32_t> tgt;
+		tgt.ensure(n);
+		k_target_of<<<nb, sblock, 0, c.stream>>>(n, g.rank.get(), T, P.d_jmap.get(), P.reach.get(), P.tmask.get(),
+												  P.amb.get(), P.x0.get(), P.Tr, tgt.get());
 		ESL_HIP(hipGetLastError());
+		std::vector<int32_t> h_tgt;
+		std::vector<double> h_x0, h_y0(P.Tr, 0.0);
+		read_host(c, h_tgt, tgt.get(), (size_t)n);
+		read_host(c, h_x0, P.x0.get(), (size_t)n);
+		for (int64_t v = 0; v < n; ++v)
+			if (h_tgt[v] >= 0)
+				h_y0[h_tgt[v]] += h_x0[v];
+		write_dev(c, P.y0.get(), h_y0);
+	}
+	// ---- exit edges of the ambiguous set, by target terminal SCC
+	P.ex_off.assign(std::max(P.Tr, 0) + 1, 0);
+	P.n_exit = 0;
+	if (P.Tr && nA)
+	{
+		sys_t S = P.sys(1, nullptr);
+		const int gA = ceil_div(nA, sblock);
+		dbuf<int32_t> cnt, ptr;
+		cnt.ensure(nA);
+		ptr.ensure(nA + 1);
+		k_exit_count<<<gA, sblock, 0, c.stream>>>(S, g.rank.get(), T, P.d_jmap.get(), P.tmask.get(), cnt.get());
+		exclusive_scan_i32(c, cnt.get(), ptr.get(), nA);
+		int32_t E = 0;
+		ESL_HIP(hipMemcpyAsync(&E, ptr.get() + nA, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
 		ESL_HIP(hipStreamSynchronize(c.stream));
+		dbuf<int32_t> ej, eu;
+		dbuf<uint8_t> er;
+		ej.ensure(std::max(E, 1));
+		eu.ensure(std::max(E, 1));
+		er.ensure(std::max(E, 1));
+		k_exit_fill<<<gA, sblock, 0, c.stream>>>(S, g.rank.get(), T, P.d_jmap.get(), P.tmask.get(), ptr.get(),
+												 ej.get(), eu.get(), er.get());
+		ESL_HIP(hipGetLastError());
+		std::vector<int32_t> h_j, h_u;
+		std::vector<uint8_t> h_r;
+		read_host(c, h_j, ej.get(), (size_t)E);
+		read_host(c, h_u, eu.get(), (size_t)E);
+		read_host(c, h_r, er.get(), (size_t)E);
+		// stable counting sort by target
+		for (int32_t e = 0; e < E; ++e)
+		{
+			if (h_j[e] < 0 || h_j[e] >= P.Tr)
+				throw error(ESL_E_DEVICE, "exit edge with a target outside the reached terminal SCCs");
+			P.ex_off[h_j[e] + 1]++;
+		}
+		for (int j = 0; j < P.Tr; ++j)
+			P.ex_off[j + 1] += P.ex_off[j];
+		std::vector<int32_t> fill(P.ex_off.begin(), P.ex_off.end() - 1), s_u(E);
+		std::vector<uint8_t> s_r(E);
+		for (int32_t e = 0; e < E; ++e)
+		{
+			const int32_t at = fill[h_j[e]]++;
+			s_u[at] = h_u[e];
+			s_r[at] = h_r[e];
+		}
+		P.ex_u.ensure(std::max(E, 1));
+		P.ex_r.ensure(std::max(E, 1));
+		P.d_ex_off.ensure(P.Tr + 1);
+		write_dev(c, P.ex_u.get(), s_u);
+		write_dev(c, P.ex_r.get(), s_r);
+		write_dev(c, P.d_ex_off.get(), P.ex_off);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		P.n_exit = E;
 	}
 	// ---- coupling structure of the per-set solver
 	P.nbr.ready = false;
@@ -4013,13 +4119,8 @@ void plan_enqueue_post(ctx_t& c, plan_t& P, int B, const double* d_rates, double
 	if (Tr)
 	{
 		if (P.nA)
-		{
-			const int nbA = ceil_div(P.nA, sblock);
-			k_exit_flow<<<dim3(nbA, B), sblock, Tr * sizeof(double), c.stream>>>(S, g.rank.get(), T, P.d_jmap.get(),
-																				 P.tmask.get(), P.rb.z, Tr,
-																				 P.rb.fpart);
-			k_reduce_parts<<<dim3(Tr, B), wave, 0, c.stream>>>(P.rb.fpart, nbA, Tr, P.rb.y, Tr, 0);
-		}
+			k_exit_sum<<<dim3(Tr, B), sblock, 0, c.stream>>>(S, P.d_ex_off.get(), P.ex_u.get(), P.ex_r.get(), P.rb.z,
+															 Tr, P.rb.y);
 		else
 			k_zero<double><<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>((int64_t)B * Tr, P.rb.y);
 		k_y_total<<<ceil_div((int64_t)B * Tr, sblock), sblock, 0, c.stream>>>(B, Tr, P.y0.get(), P.rb.y);
@@ -4086,7 +4187,7 @@ void plan_reserve(plan_t& P, int B, int m, int precision)
 	auto ptrs = [&] {
 		return std::vector<const void*>{ P.p0.get(),	 P.p1.get(),	  P.anchor.get(), P.invd.get(),
 										 P.rhs.get(),	 P.z.get(),		  P.y.get(),	  P.segsum.get(),
-										 P.fpart.get(), P.scr.r.get(),	  P.scr.udiag.get(), P.scr.res.get(),
+										 P.scr.r.get(),	  P.scr.udiag.get(), P.scr.res.get(),
 										 P.scr.t.get(), P.scr.its.get(), P.scr.cyc.get(), P.scr.stl.get(),
 										 P.scr.Vf.get(), P.scr.Vd.get() };
 	};
@@ -4107,7 +4208,6 @@ void plan_reserve_(plan_t& P, int B, int m, int precision)
 	P.z.ensure((size_t)B * nU);
 	P.y.ensure((size_t)B * std::max(P.Tr, 1));
 	P.segsum.ensure((size_t)B * std::max(P.n_seg, 1));
-	P.fpart.ensure((size_t)B * std::max(ceil_div(std::max<int64_t>(P.nA, 1), sblock), 1) * std::max(P.Tr, 1));
 	P.scr.reserve(B, nU, m, precision);
 	P.use_own_bufs();
 }
@@ -4480,7 +4580,6 @@ plan_t::run_bufs_t carve_run_bufs(const plan_t& P, int B, int m, carve_t& cv)
 {
 	const size_t nU = (size_t)std::max<int64_t>(P.nU, 1), nUt = (size_t)std::max<int64_t>(P.nUt, 1);
 	const size_t nseg = (size_t)std::max(P.n_seg, 1), Tr = (size_t)std::max(P.Tr, 1);
-	const size_t nbA = (size_t)std::max(ceil_div(std::max<int64_t>(P.nA, 1), sblock), 1);
 	plan_t::run_bufs_t rb;
 	rb.p0 = cv.take<double>(B * nUt);
 	rb.p1 = cv.take<double>(B * nUt);
@@ -4490,7 +4589,6 @@ plan_t::run_bufs_t carve_run_bufs(const plan_t& P, int B, int m, carve_t& cv)
 	rb.z = cv.take<double>(B * nU);
 	rb.y = cv.take<double>(B * Tr);
 	rb.segsum = cv.take<double>(B * nseg);
-	rb.fpart = cv.take<double>(B * nbA * Tr);
 	rb.ps.r = cv.take<double>(B * nU);
 	rb.ps.udiag = cv.take<double>(B * nU);
 	rb.ps.t = cv.take<double>(B * nU);
@@ -4818,9 +4916,11 @@ void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final,
 
 // Times candidate schedules of a plan on the caller's own batch and keeps the
 // fastest one that meets the tolerance: per-set LDS-resident / per-set from
-// global memory / device-wide, restart length 16 / 32 / 64.  Each candidate
-// runs once untimed (graph capture, buffer growth) and twice timed (best of
-// two).  Later solves use the choice when they ask for restart 0.
+// global memory / device-wide / per-set LDS-resident with the basis in global
+// memory, restart length 16 / 24 / 32 / 64.  Each candidate runs once untimed
+// (graph capture, buffer growth) and three times timed (best of three).  The
+// chosen schedule then runs once more, so d_final holds its results.  Later
+// solves use the choice when they ask for restart 0.
 void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
 				   int32_t choice[2], double* best_ms)
 {
@@ -4886,6 +4986,12 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 		throw error(ESL_E_NUMERIC, "autotune: no schedule reached the tolerance");
 	}
 	P.tune = best_t;
+	{
+		esl_solve_report rep{};
+		esl_solve_opts of = o_in;
+		of.restart = 0; // the tuned schedule, with the caller's fallback setting
+		plan_run(c, P, B, d_rates, d_final, of, rep);
+	}
 	choice[0] = P.tune.path;
 	choice[1] = P.tune.m;
 	*best_ms = best;
@@ -5057,8 +5163,9 @@ void solution_full(ctx_t& c, graph_t& g, const double* d_rates, const esl_solve_
 		return;
 	}
 	const bool fallback = T > max_rhs_terms;
-	if (T > 4096)
-		throw error(ESL_E_ARG, "full solution export supports at most 4096 terminal SCCs");
+	if (T > 65535)
+		throw error(ESL_E_ARG, "full solution export supports at most 65535 terminal SCCs (one right-hand side "
+							   "per terminal SCC in one launch dimension)");
 	dbuf<uint8_t> reach;
 	reach.ensure(n);
 	k_fill_u8<<<nb, sblock, 0, c.stream>>>(n, 1, reach.get());

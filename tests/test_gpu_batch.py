@@ -406,3 +406,93 @@ def test_exact_fallback_refuses_blocks_beyond_its_limit(ctx):
     assert "exceeds" in str(e.value)
     res, its, meth = plan.set_results(B)
     assert (meth == -1).all()
+
+
+def latch_bnet(n_latch):
+    """n_latch memory nodes m_i = m_i | t and a trigger t = false: while t is on,
+    any memory node may switch on, and t switches off for good.  The 2^n_latch
+    states with t off are fixed points (terminal SCCs); a state with t on and
+    memory pattern M reaches every fixed point whose pattern contains M."""
+    lines = ["targets, factors"]
+    for i in range(n_latch):
+        lines.append(f"m{i}, m{i} | t")
+    lines.append("t, t & !t")
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("n_latch", [5, 6, 7, 9])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_many_terminal_sccs_match_oracle(ctx, n_latch, precision):
+    # 32, 64 (every bit of the terminal masks), 128 and 512 reachable terminal
+    # SCCs; beyond 64 every reachable transient state enters the absorption system
+    text = latch_bnet(n_latch)
+    m = model_builder().parse_model(text)
+    N = n_latch + 1
+    n = 1 << N
+    B = 3
+    rng = np.random.default_rng(n_latch)
+    rates = np.exp(rng.uniform(np.log(0.1), np.log(10), (B, 2 * N)))
+    t = transition_table(ctx, m, rates[0])
+    t.construct_table()
+    g = transition_graph(ctx, t)
+    g.reorganize_vertices()
+    assert g.terminals_count == 1 << n_latch
+    x0 = initial_state(m.nodes).state
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    assert plan.info()["terminals_reached"] == 1 << n_latch
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=precision, tol=1e-13 if precision == 0 else 1e-12)
+    assert rep["exact_solves"] == 0
+    xb = dout.download()
+    om = read_bnet_text(text)
+    for k in range(B):
+        ip, rows, _ = P.transition_table(om, rates[k])
+        gr = P.transition_graph(n, ip, rows)
+        ref = P.solve(n, ip, rows, rates[k], x0, gr, full=False).final_state
+        bar = 1e-12 if precision == 0 else 1e-10
+        assert np.abs(xb[k] - ref).sum() <= bar, (k, np.abs(xb[k] - ref).sum())
+
+
+@pytest.mark.parametrize("case", ["c0-10", "c2-37", "latch-7", "uneven-15"])
+def test_results_are_bitwise_reproducible(ctx, case):
+    # two plans built from scratch, each solved twice (once replayed as a HIP
+    # graph): every final vector bit for bit the same -- no floating-point
+    # accumulation depends on the order in which threads or blocks run
+    B = 6
+    if case.startswith("latch"):
+        text = latch_bnet(int(case.split("-")[1]))
+        m = model_builder().parse_model(text)
+        N = len(m.nodes)
+        rates = np.exp(np.random.default_rng(1).uniform(np.log(0.1), np.log(10), (B, 2 * N)))
+        x0 = initial_state(m.nodes).state
+        t = transition_table(ctx, m, rates[0])
+        t.construct_table()
+        g = transition_graph(ctx, t)
+        g.reorganize_vertices()
+    elif case.startswith("uneven"):
+        net, rng = synth.uneven(0)[3]
+        rates = synth.rate_sets(rng, net.N, B)
+        m = model_builder().parse_model(net.bnet())
+        t = transition_table(ctx, m, rates[0])
+        t.construct_table()
+        g = transition_graph(ctx, t)
+        g.reorganize_vertices()
+        x0 = initial_state(m.nodes).state
+    else:
+        cfg, seed = case.split("-")
+        net, rates, m, t, g, x0 = setup_net(ctx, cfg, int(seed), B)
+    n = len(x0)
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    outs = []
+    for build in range(2):
+        plan = solve_plan(ctx, t, dx0.ptr)
+        for graph in (0, 1):
+            d = DeviceArray(ctx, (B, n), np.float64)
+            plan.solve(B, dr.ptr, d.ptr, precision=1, tol=1e-12, graph=graph)
+            outs.append(d.download())
+        plan.close()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
