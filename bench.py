@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sets", type=int, default=1024)
-    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=None,
+                    help="network seed (default: 0 for n20 and uneven, 11 for n16 and warm, 8 for n22)")
     ap.add_argument("--plain-sets", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cap", type=float, default=20.0,
@@ -250,6 +251,8 @@ def main():
         WORKLOADS[a.workload](a, cu_context(0))
         return
 
+    if a.seed is None:
+        a.seed = 0
     net, rng = synth.config("c1", a.seed)
     N = net.N
     n = 1 << N
@@ -594,7 +597,7 @@ def workload_n16(a, ctx):
     from paraexastolog_amd import synth
     from paraexastolog_amd.surrogate import DeviceArray
 
-    seed = 11 if a.seed == 0 else a.seed
+    seed = 11 if a.seed is None else a.seed
     net, rng = synth.config("c2", seed)
     B = a.sets
     rates = synth.rate_sets(rng, net.N, B)
@@ -639,6 +642,7 @@ def workload_uneven(a, ctx):
     from paraexastolog_amd.surrogate import DeviceArray
 
     per = 128
+    a.seed = 0 if a.seed is None else a.seed
     nets = synth.uneven(a.seed)
     tops, drs, dfs, rates_all = [], [], [], []
     for net, rng in nets:
@@ -677,7 +681,7 @@ def workload_n22(a, ctx):
     from paraexastolog_amd import synth
     from paraexastolog_amd.surrogate import DeviceArray
 
-    seed = 8 if a.seed == 0 else a.seed
+    seed = 8 if a.seed is None else a.seed
     net, rng = synth.config("c3", seed)
     B = min(a.sets, 64)
     rates = synth.rate_sets(rng, net.N, B)
@@ -719,7 +723,7 @@ def workload_warm(a, ctx):
     from paraexastolog_amd import synth
     from paraexastolog_amd.surrogate import DeviceArray, Surrogate, log_rates, terminal_targets, train
 
-    seed = 11 if a.seed == 0 else a.seed
+    seed = 11 if a.seed is None else a.seed
     net, rng = synth.config("c2", seed)
     n_train, n_test = 50000, 4096
     rates = synth.rate_sets(rng, net.N, n_train + n_test)

@@ -8,7 +8,10 @@
 #include <string>
 #include <vector>
 #include "esl.h"
+#include "esl_symb.hpp"
+#include <algorithm>
 #include <cstdint>
+#include <iostream>
 #include <memory>
 #include <utility>
 
@@ -61,10 +64,17 @@ struct initial_state {                    // initial_state.h:5-11
         ok(esl_initial_state(m.h, f.data(), v.data(), (int)f.size(), p, state.data()));
     }
 };
+struct transition_table;
+// the tables alive in this thread: transition_graph(ctx, rows, cols, indptr)
+// finds the table that owns the vectors it is given, as the reference keeps
+// const references to them (transition_graph.h:10)
+inline std::vector<transition_table*>& live_tables() { static thread_local std::vector<transition_table*> v; return v; }
 struct transition_table {                 // transition_table.h:7-26
     cu_context& ctx; const model_t& model; std::vector<double> r; esl_graph* g{};
     std::vector<int32_t> indptr, rows, cols;
-    transition_table(cu_context& c, const model_t& m, const std::vector<double>& rates) : ctx(c), model(m), r(rates) {}
+    transition_table(cu_context& c, const model_t& m, const std::vector<double>& rates) : ctx(c), model(m), r(rates) {
+        live_tables().push_back(this);
+    }
     void construct_table() {
         ok(esl_table_build(ctx.h, model.h, r.data(), &g), ctx.h);
         int64_t s[7]; ok(esl_graph_query(g, s));
@@ -73,26 +83,140 @@ struct transition_table {                 // transition_table.h:7-26
     }
     transition_table(const transition_table&) = delete;
     transition_table& operator=(const transition_table&) = delete;
-    ~transition_table() { esl_graph_destroy(g); }
+    ~transition_table() {
+        auto& v = live_tables();
+        v.erase(std::remove(v.begin(), v.end(), this), v.end());
+        esl_graph_destroy(g);
+    }
 };
 struct transition_graph {                 // transition_graph.h:6-28
     cu_context& ctx; transition_table& t;
     int32_t terminals_count{}; std::vector<int32_t> sccs_offsets, reordered_vertices;
     transition_graph(cu_context& c, transition_table& tab) : ctx(c), t(tab) {}
+    // the reference's signature (transition_graph.h:21): the table's own vectors
+    transition_graph(cu_context& c, const std::vector<int32_t>& rows, const std::vector<int32_t>& cols,
+                     const std::vector<int32_t>& indptr) : ctx(c), t(owner(rows, cols, indptr)) {}
     void reorganize_vertices() {
         ok(esl_graph_reorganize(ctx.h, t.g), ctx.h);
         int64_t s[7]; ok(esl_graph_query(t.g, s));
         reordered_vertices.resize(s[0]); sccs_offsets.resize(s[2] + 1);
         ok(esl_graph_export(ctx.h, t.g, reordered_vertices.data(), sccs_offsets.data(), &terminals_count, nullptr), ctx.h);
     }
-};
-struct solver {                           // solver.h:10-67
-    cu_context& ctx; transition_table& t; std::vector<double> rates, x0, final_state;
-    solver(cu_context& c, transition_table& tab, const transition_graph&, const transition_rates& r, const initial_state& s)
-        : ctx(c), t(tab), rates(r.rates), x0(s.state), final_state(s.state.size()) {}
-    void solve() {
-        esl_solve_opts o; esl_solve_opts_default(&o);
-        ok(esl_solve(ctx.h, t.g, rates.data(), x0.data(), final_state.data(), &o, nullptr), ctx.h);
+private:
+    static transition_table& owner(const std::vector<int32_t>& rows, const std::vector<int32_t>& cols,
+                                   const std::vector<int32_t>& indptr) {
+        for (transition_table* tab : live_tables())
+            if (&tab->rows == &rows && &tab->cols == &cols && &tab->indptr == &indptr) return *tab;
+        throw std::runtime_error("transition_graph: rows, cols and indptr must be the members of a live transition_table");
     }
+};
+template <bool CSR>
+struct sparse_cs_matrix {                 // sparse_utils.h:12-50 (host vectors, FP64 values)
+    std::vector<int32_t> indptr, indices;
+    std::vector<double> data;
+    int64_t nnz() const { return (int64_t)indices.size(); }
+    int64_t n() const { return indptr.empty() ? 0 : (int64_t)indptr.size() - 1; }
+};
+using sparse_csc_matrix = sparse_cs_matrix<false>;
+using sparse_csr_matrix = sparse_cs_matrix<true>;
+using persistent_data = esl_symb::data_t; // persistent_solution.h:5-24 (the .symb file's arrays)
+
+struct solver {                           // solver.h:10-67
+    cu_context& ctx; transition_table* t{}; const transition_graph* gr{};
+    const model_t& model; std::vector<double> rates, x0;
+    persistent_data* persisted{};
+    sparse_csc_matrix solution_term;      // n x T: terminal SCC j's kernel in column j
+    sparse_csr_matrix solution_nonterm;   // T x n: absorption probabilities
+    std::vector<double> final_state;
+    bool recompute_needed = false;
+    solver(cu_context& c, transition_table& tab, const transition_graph& g, const transition_rates& r, const initial_state& s)
+        : ctx(c), t(&tab), gr(&g), model(r.model), rates(r.rates), x0(s.state), final_state(s.state.size()) {}
+    // solver(ctx, persisted, rates, state) (solver.h:49, solver.cu:25-51): with the
+    // stored rates the stored solution is reused; otherwise solve() recomputes on
+    // the stored table and graph stage (same zero pattern required)
+    solver(cu_context& c, persistent_data& p, const transition_rates& r, const initial_state& s)
+        : ctx(c), model(r.model), rates(r.rates), x0(s.state), persisted(&p), final_state(s.state.size()) {
+        recompute_needed = !esl_symb::same_rates(p, rates);
+        if (!recompute_needed) {
+            solution_term = {p.term_indptr, p.term_indices, std::vector<double>(p.term_data.begin(), p.term_data.end())};
+            solution_nonterm = {p.nonterm_indptr, p.nonterm_indices,
+                                std::vector<double>(p.nonterm_data.begin(), p.nonterm_data.end())};
+        }
+    }
+    ~solver() { if (own) esl_graph_destroy(own); }
+    solver(const solver&) = delete;
+    solver& operator=(const solver&) = delete;
+    void solve() {
+        if (persisted && !recompute_needed) {       // R (L x0) from the stored solution (solver.cu:302-309)
+            final_state = esl_symb::final_from_solution(*persisted, x0);
+            return;
+        }
+        esl_graph* g = t ? t->g : nullptr;
+        if (persisted) {                            // stored graph stage, new rates of the same zero pattern
+            if (!esl_symb::compatible_zero_rates(*persisted, rates))
+                throw std::runtime_error("rates are not compatible with the stored symbolic data");
+            ok(esl_table_build(ctx.h, model.h, rates.data(), &own), ctx.h);
+            std::vector<int32_t> offs(persisted->terminals_offsets);
+            offs.insert(offs.end(), persisted->nonterminals_offsets.begin() + 1, persisted->nonterminals_offsets.end());
+            ok(esl_graph_import(ctx.h, own, persisted->ordered_vertices.data(), offs.data(), (int64_t)offs.size() - 1,
+                                (int32_t)persisted->terminals_offsets.size() - 1), ctx.h);
+            g = own;
+        }
+        esl_solve_opts o; esl_solve_opts_default(&o);
+        o.full = 1;                                 // solution_term / solution_nonterm, as solve() always builds them
+        ok(esl_solve(ctx.h, g, rates.data(), x0.data(), final_state.data(), &o, nullptr), ctx.h);
+        int64_t sz[7]; ok(esl_graph_query(g, sz));
+        const int64_t T = sz[3], n_t = sz[4];
+        int64_t nnz_l = 0;
+        ok(esl_solution_export(ctx.h, g, nullptr, nullptr, nullptr, &nnz_l, nullptr, nullptr, nullptr), ctx.h);
+        solution_term = {std::vector<int32_t>(T + 1), std::vector<int32_t>(n_t), std::vector<double>(n_t)};
+        solution_nonterm = {std::vector<int32_t>(T + 1), std::vector<int32_t>(nnz_l), std::vector<double>(nnz_l)};
+        ok(esl_solution_export(ctx.h, g, solution_term.indptr.data(), solution_term.indices.data(),
+                               solution_term.data.data(), &nnz_l, solution_nonterm.indptr.data(),
+                               solution_nonterm.indices.data(), solution_nonterm.data.data()), ctx.h);
+    }
+    // solver.cu:336-372: non-zero states (as stored, FP32) as their nodes joined by " --- ",
+    // "<nil>" for state 0, then the value with the stream's default precision
+    void print_final_state(const std::vector<std::string>& model_nodes, std::ostream& os = std::cout) const {
+        for (size_t s = 0; s < final_state.size(); ++s) {
+            const float v = (float)final_state[s];
+            if (v == 0.0f) continue;
+            bool any = false;
+            for (size_t i = 0; i < model_nodes.size(); ++i)
+                if (s & (size_t(1) << i)) { os << (any ? " --- " : "") << model_nodes[i]; any = true; }
+            if (!any) os << "<nil>";
+            os << " " << v << std::endl;
+        }
+    }
+    friend struct persistent_solution;
+private:
+    esl_graph* own{};                     // the table rebuilt for a persisted solve
+};
+
+struct persistent_solution {              // persistent_solution.h:27-37 (.symb, esl_symb.hpp)
+    static persistent_data deserialize(const std::string& file, bool no_inverse) {
+        return esl_symb::read(file, !no_inverse);
+    }
+    // the table, graph stage, rates and solution of a solved solver; no inverse is
+    // written (this library factors nothing), as with no_inverse = true
+    static void serialize(const std::string& file, const solver& s, bool /*no_inverse*/) {
+        if (!s.t || !s.gr) throw std::runtime_error("serialize: the solver was not built from a table and a graph");
+        persistent_data d;
+        d.rows = s.t->rows; d.cols = s.t->cols; d.indptr = s.t->indptr;
+        d.ordered_vertices = s.gr->reordered_vertices;
+        const size_t T = (size_t)s.gr->terminals_count;
+        d.terminals_offsets.assign(s.gr->sccs_offsets.begin(), s.gr->sccs_offsets.begin() + T + 1);
+        d.nonterminals_offsets.assign(s.gr->sccs_offsets.begin() + T, s.gr->sccs_offsets.end());
+        for (double v : s.rates) d.rates.push_back((float)v);
+        d.term_indptr = s.solution_term.indptr; d.term_indices = s.solution_term.indices;
+        for (double v : s.solution_term.data) d.term_data.push_back((float)v);
+        d.nonterm_indptr = s.solution_nonterm.indptr; d.nonterm_indices = s.solution_nonterm.indices;
+        for (double v : s.solution_nonterm.data) d.nonterm_data.push_back((float)v);
+        esl_symb::write(file, d);
+    }
+    static bool has_compatible_zero_rates(const persistent_data& d, const std::vector<double>& r) {
+        return esl_symb::compatible_zero_rates(d, r);
+    }
+    static bool are_same(const persistent_data& d, const std::vector<double>& r) { return esl_symb::same_rates(d, r); }
 };
 } // namespace compat

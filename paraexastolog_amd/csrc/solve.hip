@@ -416,74 +416,6 @@ __global__ __launch_bounds__(wave) void k_reduce_parts(const double* __restrict_
 		out[(int64_t)b * ostride + k] = accumulate ? out[(int64_t)b * ostride + k] + s : s;
 }
 
-// partial dots: part[b][blk][i] = sum_u W[b][u] * V_i[b][u], i = 0..K-1
-template <typename T, int KMAX>
-__global__ __launch_bounds__(sblock) void k_dots(int64_t nU, int B, int K, const T* __restrict__ W,
-												 const T* __restrict__ V, int64_t vstride,
-												 double* __restrict__ part, const int32_t* __restrict__ live)
-{
-	__shared__ double s_red[sblock / wave + 1];
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (!live[b]) // the whole block belongs to one set: uniform exit before any barrier
-		return;
-	double w = 0.0;
-	if (u < nU)
-		w = (double)W[(int64_t)b * nU + u];
-	for (int i = 0; i < K; ++i)
-	{
-		double d = 0.0;
-		if (u < nU)
-			d = w * (double)V[i * vstride + (int64_t)b * nU + u];
-		d = block_sum<sblock>(d, s_red);
-		if (threadIdx.x == 0)
-			part[((int64_t)b * gridDim.x + blockIdx.x) * K + i] = d;
-	}
-}
-
-// W -= sum_i c[b][i] V_i (c[b] at c + b * cstride); partial sums of |W|^2
-template <typename T>
-__global__ __launch_bounds__(sblock) void k_orth_update(int64_t nU, int K, T* __restrict__ W,
-														const T* __restrict__ V, int64_t vstride,
-														const double* __restrict__ c, int64_t cstride,
-														double* __restrict__ part, const int32_t* __restrict__ live)
-{
-	__shared__ double s_red[sblock / wave + 1];
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (!live[b]) // uniform per block
-		return;
-	double w2 = 0.0;
-	if (u < nU)
-	{
-		double w = (double)W[(int64_t)b * nU + u];
-		const double* cb = c + (int64_t)b * cstride;
-		for (int i = 0; i < K; ++i)
-			w -= cb[i] * (double)V[i * vstride + (int64_t)b * nU + u];
-		const T wt = (T)w;
-		W[(int64_t)b * nU + u] = wt;
-		w2 = (double)wt * (double)wt;
-	}
-	w2 = block_sum<sblock>(w2, s_red);
-	if (threadIdx.x == 0)
-		part[(int64_t)b * gridDim.x + blockIdx.x] = w2;
-}
-
-// V_next = W / h  (h = 0 leaves V_next = 0)
-template <typename T>
-__global__ __launch_bounds__(sblock) void k_scale(int64_t nU, const T* __restrict__ W,
-												  const double* __restrict__ hnorm, T* __restrict__ Vn,
-												  const int32_t* __restrict__ live)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= nU || !live[b])
-		return;
-	const double h = hnorm[b];
-	const double inv = h > 0.0 ? 1.0 / h : 0.0;
-	Vn[(int64_t)b * nU + u] = (T)((double)W[(int64_t)b * nU + u] * inv);
-}
-
 // per-set GMRES state (one thread per set)
 struct gm_state_t
 {
@@ -603,38 +535,6 @@ __global__ void k_gm_solve_y(int B, int m, int k, gm_state_t st)
 	}
 	for (int i = k; i <= m; ++i)
 		y[i] = 0.0;
-}
-
-// z += sum_i y_i V_i
-template <typename T, typename Z>
-__global__ __launch_bounds__(sblock) void k_combine(int64_t nU, int k, const T* __restrict__ V, int64_t vstride,
-													const double* __restrict__ y, int m, Z* __restrict__ z,
-													const int32_t* __restrict__ kb)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= nU)
-		return;
-	k = min(k, kb[b]);
-	if (k == 0)
-		return;
-	double acc = (double)z[(int64_t)b * nU + u];
-	const double* yb = y + (int64_t)b * (m + 1);
-	for (int i = 0; i < k; ++i)
-		acc += yb[i] * (double)V[i * vstride + (int64_t)b * nU + u];
-	z[(int64_t)b * nU + u] = (Z)acc;
-}
-
-template <typename T>
-__global__ __launch_bounds__(sblock) void k_cast_scale(int64_t nU, const double* __restrict__ r,
-													   const double* __restrict__ scale, T* __restrict__ out)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= nU)
-		return;
-	const double s = scale[b];
-	out[(int64_t)b * nU + u] = (T)(s > 0.0 ? r[(int64_t)b * nU + u] / s : 0.0);
 }
 
 __global__ __launch_bounds__(sblock) void k_fill_value(int64_t count, double v, double* __restrict__ p)
@@ -2388,140 +2288,7 @@ __global__ __launch_bounds__(pblock) void k_anchor_lvl(sys_t S, nbr_t G, const i
 }
 
 // ============================================================================
-// device-wide GMRES kernels (nbr_t coupling lists, iterate x unpreconditioned)
-
-__device__ __forceinline__ const double* set_rates(const sys_t& S, int b)
-{
-	return S.rates + (int64_t)b * S.rate_stride;
-}
-
-template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_apply(sys_t S, nbr_t G, int n_seg, const double* __restrict__ tin,
-													 X* __restrict__ W, const int32_t* __restrict__ live)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= S.nU || !live[b])
-		return;
-	const int64_t off = (int64_t)b * S.nU;
-	W[off + u] = (X)row_apply<double>(S, G, n_seg, b, u, tin + off, set_rates(S, b));
-}
-
-// r = rhs - M x (FP64) and partial sums of r^2, rhs^2
-__global__ __launch_bounds__(sblock) void k_mk_residual(sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs,
-														const double* __restrict__ x, double* __restrict__ r,
-														double* __restrict__ part)
-{
-	__shared__ double s_red[2][sblock / wave + 1];
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	double rr = 0.0, bb = 0.0;
-	if (u < S.nU)
-	{
-		const int64_t off = (int64_t)b * S.nU;
-		const double bv = rhs[off + u];
-		const double res = bv - row_apply<double>(S, G, n_seg, b, u, x + off, set_rates(S, b));
-		r[off + u] = res;
-		rr = res * res;
-		bb = bv * bv;
-	}
-	rr = block_sum<sblock>(rr, s_red[0]);
-	bb = block_sum<sblock>(bb, s_red[1]);
-	if (threadIdx.x == 0)
-	{
-		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 0] = rr;
-		part[((int64_t)b * gridDim.x + blockIdx.x) * 2 + 1] = bb;
-	}
-}
-
-template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_jacobi(sys_t S, X* __restrict__ v, const int32_t* __restrict__ live)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= S.nU || !live[b])
-		return;
-	const int64_t i = (int64_t)b * S.nU + u;
-	v[i] = (X)((double)v[i] * S.invd[i]);
-}
-
-// ILU(0) pivots of one popcount level (see k_gmres_set)
-__global__ __launch_bounds__(sblock) void k_mk_ilu_pivot(sys_t S, nbr_t G, int n_seg, int l, double* __restrict__ ud)
-{
-	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (q >= G.L.off[l + 1])
-		return;
-	const int32_t u = G.lvl[q];
-	const int64_t off = (int64_t)b * S.nU;
-	const double* r = set_rates(S, b);
-	double d = 1.0 / S.invd[off + u];
-	if (!is_anchor(S, n_seg, b, u))
-		for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
-		{
-			const int code = G.nr[k];
-			const int32_t uk = G.nu[k];
-			if ((code & nbr_rev_flag) && !is_anchor(S, n_seg, b, uk))
-				d -= r[code & 63] * r[(code & 63) ^ 1] / ud[off + uk];
-		}
-	ud[off + u] = d;
-}
-
-template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_ilu_fwd(sys_t S, nbr_t G, int n_seg, int l,
-													   const double* __restrict__ ud, X* __restrict__ v,
-													   const int32_t* __restrict__ live)
-{
-	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (q >= G.L.off[l + 1] || !live[b])
-		return;
-	const int32_t u = G.lvl[q];
-	if (is_anchor(S, n_seg, b, u))
-		return;
-	const int64_t off = (int64_t)b * S.nU;
-	const double* r = set_rates(S, b);
-	double y = (double)v[off + u];
-	for (int32_t k = G.ptr[u]; k < G.mid[u]; ++k)
-	{
-		const int32_t uk = G.nu[k];
-		y += r[G.nr[k] & 63] / ud[off + uk] * (double)v[off + uk];
-	}
-	v[off + u] = (X)y;
-}
-
-template <typename X>
-__global__ __launch_bounds__(sblock) void k_mk_ilu_bwd(sys_t S, nbr_t G, int n_seg, int l,
-													   const double* __restrict__ ud, X* __restrict__ v,
-													   const int32_t* __restrict__ live)
-{
-	const int q = G.L.off[l] + blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (q >= G.L.off[l + 1] || !live[b])
-		return;
-	const int32_t u = G.lvl[q];
-	if (is_anchor(S, n_seg, b, u))
-		return;
-	const int64_t off = (int64_t)b * S.nU;
-	const double* r = set_rates(S, b);
-	double z = (double)v[off + u];
-	for (int32_t k = G.mid[u]; k < G.ptr[u + 1]; ++k)
-		z += r[G.nr[k] & 63] * (double)v[off + G.nu[k]];
-	v[off + u] = (X)(z / ud[off + u]);
-}
-
-// b[set][u] = a[set][u] for the sets still iterating (grid: unknowns x sets)
-template <typename T>
-__global__ __launch_bounds__(sblock) void k_copy_live(int64_t nU, const T* __restrict__ a, double* __restrict__ out,
-													  const int32_t* __restrict__ live)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= nU || !live[b])
-		return;
-	out[(int64_t)b * nU + u] = (double)a[(int64_t)b * nU + u];
-}
-
+// element-wise updates of the device-wide solver (any layout)
 
 __global__ __launch_bounds__(sblock) void k_axpy1(int64_t count, const double* __restrict__ a, double* __restrict__ x)
 {
@@ -2556,6 +2323,512 @@ __global__ __launch_bounds__(sblock) void k_scale_invd(int64_t count, const doub
 	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	if (i < count)
 		v[i] = divide ? v[i] / invd[i] : v[i] * invd[i];
+}
+
+// ============================================================================
+// Device-wide GMRES, set-minor layout.  Every vector is [row][set]: the B
+// values of one row sit side by side, and rows are numbered in popcount-level
+// order (row q = unknown lvl[q]), so a level is one contiguous range of rows.
+// A row kernel maps consecutive lanes to the sets of one row (then the next
+// row): the row's coupling list is read once per row (the same address for
+// all its lanes) and each neighbour's values for all sets are one contiguous
+// load -- in the set-major layout the same neighbour read was one isolated
+// 8-byte access per set.  Per-set reductions use blocks of R * B threads
+// (B <= 1024) with thread t bound to set t % B for the whole grid-stride loop,
+// and fixed-order sums: results do not depend on scheduling.
+
+// coupling lists with rows and neighbours in level order
+struct wnbr_t
+{
+	const int32_t* ptr; // [nU + 1]
+	const int32_t* mid; // [nU] end of the lower (previous level) part
+	const int32_t* nu;	// [nnz] neighbour row (level order)
+	const uint8_t* nr;	// [nnz] rate index | mirrored flag
+	const int32_t* lvl; // [nU] row -> unknown
+	levels_t L;
+};
+
+__global__ __launch_bounds__(sblock) void k_w_inv(int64_t nU, const int32_t* __restrict__ lvl, int32_t* __restrict__ inv)
+{
+	const int64_t q = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (q < nU)
+		inv[lvl[q]] = (int32_t)q;
+}
+
+__global__ __launch_bounds__(sblock) void k_w_count(int64_t nU, nbr_t G, int32_t* __restrict__ cnt)
+{
+	const int64_t q = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (q < nU)
+	{
+		const int32_t u = G.lvl[q];
+		cnt[q] = G.ptr[u + 1] - G.ptr[u];
+	}
+}
+
+__global__ __launch_bounds__(sblock) void k_w_fill(int64_t nU, nbr_t G, const int32_t* __restrict__ inv,
+												   const int32_t* __restrict__ wptr, int32_t* __restrict__ wmid,
+												   int32_t* __restrict__ wnu, uint8_t* __restrict__ wnr)
+{
+	const int64_t q = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (q >= nU)
+		return;
+	const int32_t u = G.lvl[q];
+	const int32_t a = G.ptr[u], at = wptr[q];
+	wmid[q] = at + (G.mid[u] - a);
+	for (int32_t e = a; e < G.ptr[u + 1]; ++e)
+	{
+		wnu[at + (e - a)] = inv[G.nu[e]];
+		wnr[at + (e - a)] = G.nr[e];
+	}
+}
+
+// set-major plan vectors -> set-minor solver vectors: right-hand side, iterate
+// x = invd * z, 1/D' and the anchor flag of every (row, set)
+__global__ __launch_bounds__(sblock) void k_w_gather(sys_t S, int n_seg, const int32_t* __restrict__ lvl,
+													 const double* __restrict__ rhs, const double* __restrict__ z,
+													 double* __restrict__ wrhs, double* __restrict__ wx,
+													 double* __restrict__ winvd, uint8_t* __restrict__ wanc)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int B = S.B;
+	if (i >= S.nU * B)
+		return;
+	const int64_t q = i / B;
+	const int b = (int)(i - q * B);
+	const int32_t u = lvl[q];
+	const int64_t o = (int64_t)b * S.nU + u;
+	const double iv = S.invd[o];
+	wrhs[i] = rhs[o];
+	wx[i] = z[o] * iv;
+	winvd[i] = iv;
+	wanc[i] = is_anchor(S, n_seg, b, u) ? 1 : 0;
+}
+
+// z = x / invd back into the plan's set-major layout
+__global__ __launch_bounds__(sblock) void k_w_scatter(int64_t nU, int B, const int32_t* __restrict__ lvl,
+													  const double* __restrict__ wx, const double* __restrict__ winvd,
+													  double* __restrict__ z)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= nU * B)
+		return;
+	const int64_t q = i / B;
+	const int b = (int)(i - q * B);
+	z[(int64_t)b * nU + lvl[q]] = wx[i] / winvd[i];
+}
+
+// rows [q0, q1) x B sets as a flat range: the row and set of flat index i
+__device__ __forceinline__ void w_row_set(int64_t i, int B, int64_t q0, int64_t& q, int& b)
+{
+	const int64_t rr = i / B;
+	b = (int)(i - rr * B);
+	q = q0 + rr;
+}
+
+// rates transposed for the set-minor kernels: rt[code * B + b] = rate `code`
+// of set b, so the lanes of a row (consecutive sets) read consecutive words
+__global__ __launch_bounds__(sblock) void k_w_rates(const double* __restrict__ rates, int64_t rstride, int ncode,
+													int B, double* __restrict__ rt)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= ncode * B)
+		return;
+	const int code = i / B, b = i - code * B;
+	rt[i] = rates[(int64_t)b * rstride + code];
+}
+
+// y = M x for (row q, set b): D'_u x - sum rate x_k; anchor rows give x
+template <typename X>
+__device__ __forceinline__ double w_row_apply(const wnbr_t& G, int B, int64_t q, int b, const double* __restrict__ rt,
+											  const double* __restrict__ winvd, const uint8_t* __restrict__ wanc,
+											  const X* __restrict__ x)
+{
+	const int64_t i = q * B + b;
+	if (wanc[i])
+		return (double)x[i];
+	double acc = (double)x[i] / winvd[i];
+	int32_t e = G.ptr[q];
+	const int32_t e1 = G.ptr[q + 1];
+	// four neighbours' loads in flight at a time; the sum keeps its order
+	for (; e + 4 <= e1; e += 4)
+	{
+		double xv[4], rv[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+		{
+			xv[k] = (double)x[(int64_t)G.nu[e + k] * B + b];
+			rv[k] = rt[(G.nr[e + k] & 63) * B + b];
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			acc -= rv[k] * xv[k];
+	}
+	for (; e < e1; ++e)
+		acc -= rt[(G.nr[e] & 63) * B + b] * (double)x[(int64_t)G.nu[e] * B + b];
+	return acc;
+}
+
+template <typename X, typename Y>
+__global__ __launch_bounds__(sblock) void k_w_apply(wnbr_t G, int64_t nU, int B, const double* __restrict__ rt,
+													const double* __restrict__ winvd,
+													const uint8_t* __restrict__ wanc, const Y* __restrict__ t,
+													X* __restrict__ W, const int32_t* __restrict__ live)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= nU * B)
+		return;
+	int64_t q;
+	int b;
+	w_row_set(i, B, 0, q, b);
+	if (!live[b])
+		return;
+	W[i] = (X)w_row_apply<Y>(G, B, q, b, rt, winvd, wanc, t);
+}
+
+__global__ __launch_bounds__(sblock) void k_w_residual(wnbr_t G, int64_t nU, int B, const double* __restrict__ rt,
+													   const double* __restrict__ winvd,
+													   const uint8_t* __restrict__ wanc,
+													   const double* __restrict__ rhs, const double* __restrict__ x,
+													   double* __restrict__ r)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= nU * B)
+		return;
+	int64_t q;
+	int b;
+	w_row_set(i, B, 0, q, b);
+	r[i] = rhs[i] - w_row_apply<double>(G, B, q, b, rt, winvd, wanc, x);
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_w_jacobi(int64_t count, int B, const double* __restrict__ winvd,
+													 X* __restrict__ v, const int32_t* __restrict__ live)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= count || !live[i % B])
+		return;
+	v[i] = (X)((double)v[i] * winvd[i]);
+}
+
+// ILU(0) pivots of the rows of one level (the previous level's are final)
+__global__ __launch_bounds__(sblock) void k_w_ilu_pivot(wnbr_t G, int B, int l, const double* __restrict__ rt,
+														const double* __restrict__ winvd,
+														const uint8_t* __restrict__ wanc, double* __restrict__ ud)
+{
+	const int64_t q0 = G.L.off[l];
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= (G.L.off[l + 1] - q0) * B)
+		return;
+	int64_t q;
+	int b;
+	w_row_set(i, B, q0, q, b);
+	const int64_t at = q * B + b;
+	double d = 1.0 / winvd[at];
+	if (!wanc[at])
+		for (int32_t e = G.ptr[q]; e < G.mid[q]; ++e)
+		{
+			const int code = G.nr[e];
+			const int64_t k = (int64_t)G.nu[e] * B + b;
+			if ((code & nbr_rev_flag) && !wanc[k])
+				d -= rt[(code & 63) * B + b] * rt[((code & 63) ^ 1) * B + b] / ud[k];
+		}
+	ud[at] = d;
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_w_ilu_fwd(wnbr_t G, int B, int l, const double* __restrict__ rt,
+													  const uint8_t* __restrict__ wanc,
+													  const double* __restrict__ ud, X* __restrict__ v,
+													  const int32_t* __restrict__ live)
+{
+	const int64_t q0 = G.L.off[l];
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= (G.L.off[l + 1] - q0) * B)
+		return;
+	int64_t q;
+	int b;
+	w_row_set(i, B, q0, q, b);
+	const int64_t at = q * B + b;
+	if (!live[b] || wanc[at])
+		return;
+	double y = (double)v[at];
+	int32_t e = G.ptr[q];
+	const int32_t e1 = G.mid[q];
+	for (; e + 4 <= e1; e += 4) // four neighbours' loads in flight; the sum keeps its order
+	{
+		double rv[4], uv[4], vv[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+		{
+			const int64_t at_k = (int64_t)G.nu[e + k] * B + b;
+			rv[k] = rt[(G.nr[e + k] & 63) * B + b];
+			uv[k] = ud[at_k];
+			vv[k] = (double)v[at_k];
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			y += rv[k] / uv[k] * vv[k];
+	}
+	for (; e < e1; ++e)
+	{
+		const int64_t k = (int64_t)G.nu[e] * B + b;
+		y += rt[(G.nr[e] & 63) * B + b] / ud[k] * (double)v[k];
+	}
+	v[at] = (X)y;
+}
+
+template <typename X>
+__global__ __launch_bounds__(sblock) void k_w_ilu_bwd(wnbr_t G, int B, int l, const double* __restrict__ rt,
+													  const uint8_t* __restrict__ wanc,
+													  const double* __restrict__ ud, X* __restrict__ v,
+													  const int32_t* __restrict__ live)
+{
+	const int64_t q0 = G.L.off[l];
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= (G.L.off[l + 1] - q0) * B)
+		return;
+	int64_t q;
+	int b;
+	w_row_set(i, B, q0, q, b);
+	const int64_t at = q * B + b;
+	if (!live[b] || wanc[at])
+		return;
+	double z = (double)v[at];
+	int32_t e = G.mid[q];
+	const int32_t e1 = G.ptr[q + 1];
+	for (; e + 4 <= e1; e += 4) // four neighbours' loads in flight; the sum keeps its order
+	{
+		double rv[4], vv[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+		{
+			rv[k] = rt[(G.nr[e + k] & 63) * B + b];
+			vv[k] = (double)v[(int64_t)G.nu[e + k] * B + b];
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			z += rv[k] * vv[k];
+	}
+	for (; e < e1; ++e)
+		z += rt[(G.nr[e] & 63) * B + b] * (double)v[(int64_t)G.nu[e] * B + b];
+	v[at] = (X)(z / ud[at]);
+}
+
+// per-set sums over rows: a block of R * B threads, thread t works for set
+// t % B on rows blockIdx.x * R + t / B + k * gridDim.x * R; the R partials of a
+// set are added in row order into part[blk][b][i]
+__device__ __forceinline__ void w_block_sets(double v, int B, int R, double* s_red, double* __restrict__ part,
+											 int64_t slot)
+{
+	s_red[threadIdx.x] = v;
+	__syncthreads();
+	if ((int)threadIdx.x < B)
+	{
+		double acc = 0.0;
+		for (int k = 0; k < R; ++k)
+			acc += s_red[threadIdx.x + k * B];
+		part[slot] = acc;
+	}
+	__syncthreads();
+}
+
+// partial sums of r^2 and rhs^2 per set: part[blk][b][2]
+__global__ __launch_bounds__(1024) void k_w_norms(int64_t nU, int B, int R, const double* __restrict__ r,
+												  const double* __restrict__ rhs, double* __restrict__ part)
+{
+	__shared__ double s_red[1024];
+	const int b = threadIdx.x % B;
+	double rr = 0.0, bb = 0.0;
+	if ((int)threadIdx.x < B * R)
+		for (int64_t q = (int64_t)blockIdx.x * R + threadIdx.x / B; q < nU; q += (int64_t)gridDim.x * R)
+		{
+			const double v = r[q * B + b], w = rhs[q * B + b];
+			rr += v * v;
+			bb += w * w;
+		}
+	const int64_t base = ((int64_t)blockIdx.x * B + b) * 2;
+	w_block_sets(rr, B, R, s_red, part, base + 0);
+	w_block_sets(bb, B, R, s_red, part, base + 1);
+}
+
+// partial dots: part[blk][b][i] = sum_rows W * V_i, i < K; eight basis
+// vectors per pass over the rows, so W is read once per eight
+template <typename T>
+__global__ __launch_bounds__(1024) void k_w_dots(int64_t nU, int B, int R, int K, const T* __restrict__ W,
+												 const T* __restrict__ V, int64_t vstride, double* __restrict__ part,
+												 const int32_t* __restrict__ live)
+{
+	constexpr int C = 8;
+	__shared__ double s_red[1024];
+	const int b = threadIdx.x % B;
+	const bool on = (int)threadIdx.x < B * R && live[b];
+	for (int i0 = 0; i0 < K; i0 += C)
+	{
+		double d[C];
+#pragma unroll
+		for (int c = 0; c < C; ++c)
+			d[c] = 0.0;
+		if (on)
+		{
+			// two rows per step: both rows' loads are in flight together
+			const int64_t step = (int64_t)gridDim.x * R;
+			int64_t q = (int64_t)blockIdx.x * R + threadIdx.x / B;
+			for (; q + step < nU; q += 2 * step)
+			{
+				const int64_t a0 = q * B + b, a1 = (q + step) * B + b;
+				const double w0 = (double)W[a0], w1 = (double)W[a1];
+#pragma unroll
+				for (int c = 0; c < C; ++c)
+					if (i0 + c < K)
+					{
+						const T* Vc = V + (int64_t)(i0 + c) * vstride;
+						d[c] += w0 * (double)Vc[a0] + w1 * (double)Vc[a1];
+					}
+			}
+			if (q < nU)
+			{
+				const int64_t a0 = q * B + b;
+				const double w0 = (double)W[a0];
+#pragma unroll
+				for (int c = 0; c < C; ++c)
+					if (i0 + c < K)
+						d[c] += w0 * (double)V[(int64_t)(i0 + c) * vstride + a0];
+			}
+		}
+#pragma unroll
+		for (int c = 0; c < C; ++c)
+			if (i0 + c < K)
+				w_block_sets(d[c], B, R, s_red, part, ((int64_t)blockIdx.x * B + b) * K + i0 + c);
+	}
+}
+
+// W -= sum_i c[b][i] V_i; partial sums of |W|^2 per set: part[blk][b]
+template <typename T>
+__global__ __launch_bounds__(1024) void k_w_orth(int64_t nU, int B, int R, int K, T* __restrict__ W,
+												 const T* __restrict__ V, int64_t vstride,
+												 const double* __restrict__ ct, double* __restrict__ part,
+												 const int32_t* __restrict__ live)
+{
+	__shared__ double s_red[1024];
+	const int b = threadIdx.x % B;
+	double w2 = 0.0;
+	if ((int)threadIdx.x < B * R && live[b])
+	{
+		// two rows per step: both rows' loads are in flight together
+		const int64_t step = (int64_t)gridDim.x * R;
+		int64_t q = (int64_t)blockIdx.x * R + threadIdx.x / B;
+		for (; q + step < nU; q += 2 * step)
+		{
+			const int64_t a0 = q * B + b, a1 = (q + step) * B + b;
+			double w0 = (double)W[a0], w1 = (double)W[a1];
+			for (int i = 0; i < K; ++i)
+			{
+				const double ci = ct[i * B + b];
+				const T* Vi = V + i * vstride;
+				w0 -= ci * (double)Vi[a0];
+				w1 -= ci * (double)Vi[a1];
+			}
+			const T t0 = (T)w0, t1 = (T)w1;
+			W[a0] = t0;
+			W[a1] = t1;
+			w2 += (double)t0 * (double)t0 + (double)t1 * (double)t1;
+		}
+		if (q < nU)
+		{
+			const int64_t at = q * B + b;
+			double w = (double)W[at];
+			for (int i = 0; i < K; ++i)
+				w -= ct[i * B + b] * (double)V[i * vstride + at];
+			const T wt = (T)w;
+			W[at] = wt;
+			w2 += (double)wt * (double)wt;
+		}
+	}
+	w_block_sets(w2, B, R, s_red, part, (int64_t)blockIdx.x * B + b);
+}
+
+// out[b * ostride + i] = sum over blocks of part[blk][b][i]  (one wave per
+// (i, b)); outt (if not null) gets the same values transposed, outt[i * B + b]
+__global__ __launch_bounds__(wave) void k_w_reduce(const double* __restrict__ part, int nblk, int B, int K,
+												   double* __restrict__ out, int64_t ostride, double* __restrict__ outt)
+{
+	const int i = blockIdx.x;
+	const int b = blockIdx.y;
+	double s = 0.0;
+	for (int k = threadIdx.x; k < nblk; k += wave)
+		s += part[((int64_t)k * B + b) * K + i];
+	s = wave_sum(s);
+	if (threadIdx.x == 0)
+	{
+		out[(int64_t)b * ostride + i] = s;
+		if (outt)
+			outt[(int64_t)i * B + b] = s;
+	}
+}
+
+// yt[j * B + b] = y[b][j], j <= m (the combination weights, transposed)
+__global__ __launch_bounds__(sblock) void k_w_transpose_y(int B, int m, const double* __restrict__ y,
+														  double* __restrict__ yt)
+{
+	const int i = blockIdx.x * sblock + threadIdx.x;
+	if (i >= B * (m + 1))
+		return;
+	const int j = i / B, b = i - j * B;
+	yt[i] = y[(int64_t)b * (m + 1) + j];
+}
+
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_w_cast_scale(int64_t count, int B, const double* __restrict__ r,
+														 const double* __restrict__ scale, T* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= count)
+		return;
+	const double s = scale[i % B];
+	out[i] = (T)(s > 0.0 ? r[i] / s : 0.0);
+}
+
+template <typename T, typename O>
+__global__ __launch_bounds__(sblock) void k_w_copy_live(int64_t count, int B, const T* __restrict__ a,
+														O* __restrict__ out, const int32_t* __restrict__ live)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i < count && live[i % B])
+		out[i] = (O)a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_w_scale(int64_t count, int B, const T* __restrict__ W,
+													const double* __restrict__ hnorm, T* __restrict__ Vn,
+													const int32_t* __restrict__ live)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= count)
+		return;
+	const int b = (int)(i % B);
+	if (!live[b])
+		return;
+	const double h = hnorm[b];
+	Vn[i] = (T)((double)W[i] * (h > 0.0 ? 1.0 / h : 0.0));
+}
+
+// z += sum_{i < k_b} y[b][i] V_i  (yt = y transposed: yt[i * B + b])
+template <typename T>
+__global__ __launch_bounds__(sblock) void k_w_combine(int64_t count, int B, int k, const T* __restrict__ V,
+													  int64_t vstride, const double* __restrict__ yt,
+													  double* __restrict__ z, const int32_t* __restrict__ kb)
+{
+	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	if (i >= count)
+		return;
+	const int b = (int)(i % B);
+	const int kk = min(k, kb[b]);
+	if (kk == 0)
+		return;
+	double acc = z[i];
+	for (int j = 0; j < kk; ++j)
+		acc += yt[j * B + b] * (double)V[j * vstride + i];
+	z[i] = acc;
 }
 
 // ============================================================================
@@ -3025,12 +3298,40 @@ struct nbr_store_t
 	bool packed = false; // hdr / rec built
 	levels_t L{};
 	int64_t nnz = 0;
+	// the same lists with rows and neighbours in level order (set-minor
+	// device-wide solver), built on first use
+	mutable dbuf<int32_t> wptr, wmid, wnu;
+	mutable dbuf<uint8_t> wnr;
+	mutable bool wready = false;
 	nbr_t view() const
 	{
 		return nbr_t{ ptr.get(), mid.get(), nu.get(), nr.get(), lvl.get(), L,
 					  packed ? hdr.get() : nullptr, packed ? rec.get() : nullptr };
 	}
+	wnbr_t wview() const { return wnbr_t{ wptr.get(), wmid.get(), wnu.get(), wnr.get(), lvl.get(), L }; }
 };
+
+void build_wnbr(ctx_t& c, int64_t nU, const nbr_store_t& st)
+{
+	if (st.wready)
+		return;
+	const nbr_t G = st.view();
+	const int nb = ceil_div(nU, sblock);
+	dbuf<int32_t> inv, cnt;
+	inv.ensure(nU);
+	cnt.ensure(nU);
+	st.wptr.ensure(nU + 1);
+	st.wmid.ensure(nU);
+	st.wnu.ensure(std::max<int64_t>(st.nnz, 1));
+	st.wnr.ensure(std::max<int64_t>(st.nnz, 1));
+	k_w_inv<<<nb, sblock, 0, c.stream>>>(nU, G.lvl, inv.get());
+	k_w_count<<<nb, sblock, 0, c.stream>>>(nU, G, cnt.get());
+	exclusive_scan_i32(c, cnt.get(), st.wptr.get(), nU);
+	k_w_fill<<<nb, sblock, 0, c.stream>>>(nU, G, inv.get(), st.wptr.get(), st.wmid.get(), st.wnu.get(), st.wnr.get());
+	ESL_HIP(hipGetLastError());
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	st.wready = true;
+}
 
 void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
 {
@@ -3103,23 +3404,33 @@ void build_nbr(ctx_t& c, const sys_t& S, nbr_store_t& st)
 }
 
 
-// Device-wide GMRES(m) for systems too large for one workgroup per set: the
-// same method as k_gmres_set (FP64 iterate x, right preconditioning by
-// Jacobi or by the level-scheduled ILU(0), classical Gram-Schmidt twice,
-// FP64 residual at every restart), but every vector operation is a kernel over
-// all unknowns of all sets and the ILU(0) sweeps are one kernel per popcount
-// level.  Restarts and convergence checks go through the host.
+// Device-wide GMRES(m) on the set-minor layout (the k_w_* kernels): the method
+// of gmres_t -- FP64 iterate x, right preconditioning by Jacobi or the
+// level-scheduled ILU(0), classical Gram-Schmidt (twice in FP64, once in FP32
+// cycles), FP64 residual at every restart -- with every vector [row][set] in
+// level order.  enter() gathers the plan's set-major right-hand side and
+// iterate, leave() scatters the iterate back.  At most 1024 sets (one
+// reduction block holds every set).  The preconditioned Arnoldi vector has the
+// basis type (FP32 in FP32 cycles); pivots, the iterate, residuals and the
+// correction of each cycle are FP64.
 template <typename T>
-struct gmres_t
+struct gmres_w_t
 {
 	ctx_t& c;
 	const sys_t& S;
 	const nbr_store_t& NB;
 	int n_seg, m;
 	bool ilu;
+	int64_t nU;
+	int B, R, nred;
+	wnbr_t G;
 	dbuf<T> V, W;
-	dbuf<double> t; // preconditioned vector, FP64
-	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, r, ud, res0;
+	dbuf<T> t;		  // preconditioned Arnoldi vector
+	dbuf<double> keep; // FP32 cycles: the last correction, undone if it raised the residual
+	dbuf<double> r, ud, wrhs, wx, winvd;
+	dbuf<double> rt, ct, yt; // rates, Gram-Schmidt coefficients, combination weights: [index][set]
+	dbuf<uint8_t> wanc;
+	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, res0;
 	dbuf<int32_t> live, act, kb;
 	gm_state_t st{};
 	std::vector<double> h_res;
@@ -3127,15 +3438,27 @@ struct gmres_t
 	int iters = 0, cycles = 0;
 	double final_res = 0.0;
 
-	gmres_t(ctx_t& ctx, const sys_t& sys, const nbr_store_t& nb, int nseg, int restart, bool use_ilu)
-		: c(ctx), S(sys), NB(nb), n_seg(nseg), m(restart), ilu(use_ilu)
+	gmres_w_t(ctx_t& ctx, const sys_t& sys, const nbr_store_t& nb, int nseg, int restart, bool use_ilu)
+		: c(ctx), S(sys), NB(nb), n_seg(nseg), m(restart), ilu(use_ilu), nU(sys.nU), B(sys.B)
 	{
-		const int64_t nU = S.nU, B = S.B;
-		V.ensure((size_t)(m + 1) * nU * B);
-		W.ensure((size_t)nU * B);
-		t.ensure((size_t)nU * B);
-		r.ensure((size_t)nU * B);
-		ud.ensure((size_t)nU * B);
+		if (B > 1024)
+			throw error(ESL_E_ARG, "the device-wide solver takes at most 1024 rate sets per call");
+		build_wnbr(c, nU, NB);
+		G = NB.wview();
+		R = B >= 256 ? 1 : 256 / B;
+		nred = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nU, R), 4096));
+		const size_t cnt = (size_t)nU * B;
+		V.ensure((size_t)(m + 1) * cnt);
+		W.ensure(cnt);
+		t.ensure(cnt);
+		if (sizeof(T) == 4)
+			keep.ensure(cnt);
+		r.ensure(cnt);
+		ud.ensure(cnt);
+		wrhs.ensure(cnt);
+		wx.ensure(cnt);
+		winvd.ensure(cnt);
+		wanc.ensure(cnt);
 		H.ensure((size_t)B * (m + 1) * m);
 		cs.ensure((size_t)B * m);
 		sn.ensure((size_t)B * m);
@@ -3151,9 +3474,11 @@ struct gmres_t
 		act.ensure(B);
 		kb.ensure(B);
 		res0.ensure(B);
-		const int nblk = ceil_div(nU, sblock);
-		part.ensure((size_t)B * nblk * (m + 1) + 16);
+		part.ensure((size_t)nred * B * std::max(m + 1, 2) + 16);
 		rr.ensure((size_t)B * 2);
+		rt.ensure((size_t)64 * B);
+		ct.ensure((size_t)(m + 1) * B);
+		yt.ensure((size_t)(m + 1) * B);
 		st = gm_state_t{ H.get(),	 cs.get(),	  sn.get(),	  g.get(),	 c1.get(),	c2.get(),
 						 hn.get(),	 beta.get(), bnorm.get(), res.get(), live.get(), act.get(),
 						 kb.get(),	 res0.get() };
@@ -3162,75 +3487,87 @@ struct gmres_t
 			k_zero<double><<<ceil_div(B * (m + 1), sblock), sblock, 0, c.stream>>>(B * (m + 1), c2.get());
 	}
 
+	int grid(int64_t count) const { return (int)ceil_div(count, sblock); }
+
+	// plan vectors (set-major, z = D' x) -> solver vectors; ILU(0) pivots
+	void enter(const double* rhs, const double* z)
+	{
+		k_w_gather<<<grid(nU * B), sblock, 0, c.stream>>>(S, n_seg, G.lvl, rhs, z, wrhs.get(), wx.get(), winvd.get(),
+														   wanc.get());
+		k_w_rates<<<grid(2 * S.N * B), sblock, 0, c.stream>>>(S.rates, S.rate_stride, 2 * S.N, B, rt.get());
+		if (ilu)
+			for (int l = 0; l < G.L.n; ++l)
+			{
+				const int cntl = G.L.off[l + 1] - G.L.off[l];
+				if (cntl)
+					k_w_ilu_pivot<<<grid((int64_t)cntl * B), sblock, 0, c.stream>>>(G, B, l, rt.get(), winvd.get(),
+																					 wanc.get(), ud.get());
+			}
+		ESL_HIP(hipGetLastError());
+	}
+
+	void leave(double* z)
+	{
+		k_w_scatter<<<grid(nU * B), sblock, 0, c.stream>>>(nU, B, G.lvl, wx.get(), winvd.get(), z);
+		ESL_HIP(hipGetLastError());
+	}
+
 	double max_res()
 	{
-		read_host(c, h_res, res.get(), (size_t)S.B);
+		read_host(c, h_res, res.get(), (size_t)B);
 		double mx = 0.0;
 		for (double v : h_res)
 			mx = std::max(mx, std::isnan(v) ? 1e300 : v);
 		return mx;
 	}
 
-	template <typename X>
-	void precondition(X* v, const int32_t* flags) // in place: v <- P^{-1} v for the sets with flags[b] != 0
-	{
-		const nbr_t G = NB.view();
-		const int B = S.B;
-		if (!ilu)
-		{
-			k_mk_jacobi<X><<<dim3(ceil_div(S.nU, sblock), B), sblock, 0, c.stream>>>(S, v, flags);
-			return;
-		}
-		for (int l = 0; l < G.L.n; ++l)
-		{
-			const int cnt = G.L.off[l + 1] - G.L.off[l];
-			if (cnt)
-				k_mk_ilu_fwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v,
-																						  flags);
-		}
-		for (int l = G.L.n - 1; l >= 0; --l)
-		{
-			const int cnt = G.L.off[l + 1] - G.L.off[l];
-			if (cnt)
-				k_mk_ilu_bwd<X><<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get(), v,
-																						  flags);
-		}
-	}
-
-	// true while any set still iterates in this cycle
 	bool any_live()
 	{
-		read_host(c, h_live, live.get(), (size_t)S.B);
+		read_host(c, h_live, live.get(), (size_t)B);
 		for (int32_t v : h_live)
 			if (v)
 				return true;
 		return false;
 	}
 
-	// x: FP64 iterate (in/out).  Returns false when an FP32 cycle failed to
-	// reduce the FP64 residual (the cycle is undone if it increased it); the
-	// caller then continues in FP64 from x.
-	bool solve(const double* rhs, double* x, double tol, int max_iters, double inner_rtol = 0.0)
+	template <typename X>
+	void precondition(X* v, const int32_t* flags) // in place: v <- P^{-1} v for the sets with flags[b] != 0
+	{
+		if (!ilu)
+		{
+			k_w_jacobi<X><<<grid(nU * B), sblock, 0, c.stream>>>(nU * B, B, winvd.get(), v, flags);
+			return;
+		}
+		for (int l = 0; l < G.L.n; ++l)
+		{
+			const int cntl = G.L.off[l + 1] - G.L.off[l];
+			if (cntl)
+				k_w_ilu_fwd<X><<<grid((int64_t)cntl * B), sblock, 0, c.stream>>>(G, B, l, rt.get(), wanc.get(),
+																				 ud.get(), v, flags);
+		}
+		for (int l = G.L.n - 1; l >= 0; --l)
+		{
+			const int cntl = G.L.off[l + 1] - G.L.off[l];
+			if (cntl)
+				k_w_ilu_bwd<X><<<grid((int64_t)cntl * B), sblock, 0, c.stream>>>(G, B, l, rt.get(), wanc.get(),
+																				 ud.get(), v, flags);
+		}
+	}
+
+	bool solve(double tol, int max_iters, double inner_rtol = 0.0)
 	{
 		double prev = 0.0;
-		const int64_t nU = S.nU;
-		const int B = S.B;
-		const int nblk = ceil_div(nU, sblock);
-		const dim3 grid(nblk, B);
-		const int64_t vstride = nU * B;
+		const int64_t cnt = nU * B;
+		const int gq = grid(cnt);
+		const int64_t vstride = cnt;
 		const int sb = ceil_div(B, 64);
-		const nbr_t G = NB.view();
-		if (ilu)
-			for (int l = 0; l < G.L.n; ++l)
-			{
-				const int cnt = G.L.off[l + 1] - G.L.off[l];
-				if (cnt)
-					k_mk_ilu_pivot<<<dim3(ceil_div(cnt, sblock), B), sblock, 0, c.stream>>>(S, G, n_seg, l, ud.get());
-			}
+		const int bs = B * R;
 		for (;;)
 		{
-			k_mk_residual<<<grid, sblock, 0, c.stream>>>(S, G, n_seg, rhs, x, r.get(), part.get());
-			k_reduce_parts<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nblk, 2, rr.get(), 2, 0);
+			k_w_residual<<<gq, sblock, 0, c.stream>>>(G, nU, B, rt.get(), winvd.get(), wanc.get(), wrhs.get(),
+													  wx.get(), r.get());
+			k_w_norms<<<nred, bs, 0, c.stream>>>(nU, B, R, r.get(), wrhs.get(), part.get());
+			k_w_reduce<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nred, B, 2, rr.get(), 2, nullptr);
 			k_gm_start<<<sb, 64, 0, c.stream>>>(B, m, st, rr.get(), tol, vscale.get());
 			ESL_HIP(hipGetLastError());
 			final_res = max_res();
@@ -3240,7 +3577,7 @@ struct gmres_t
 			{
 				if (!(final_res <= prev))
 				{
-					k_sub1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, t.get(), x);
+					k_sub1<<<gq, sblock, 0, c.stream>>>(cnt, keep.get(), wx.get());
 					ESL_HIP(hipGetLastError());
 					final_res = prev;
 				}
@@ -3250,42 +3587,41 @@ struct gmres_t
 			if (iters >= max_iters)
 				return false; // the caller checks every set's residual
 			++cycles;
-			k_cast_scale<T><<<grid, sblock, 0, c.stream>>>(nU, r.get(), vscale.get(), V.get());
+			k_w_cast_scale<T><<<gq, sblock, 0, c.stream>>>(cnt, B, r.get(), vscale.get(), V.get());
 			int k = 0;
 			for (int j = 0; j < m; ++j)
 			{
 				T* Vj = V.get() + (int64_t)j * vstride;
-				k_copy_live<T><<<grid, sblock, 0, c.stream>>>(nU, Vj, t.get(), live.get());
-				precondition<double>(t.get(), live.get());
+				k_w_copy_live<T, T><<<gq, sblock, 0, c.stream>>>(cnt, B, Vj, t.get(), live.get());
+				precondition<T>(t.get(), live.get());
 				// algorithmic bytes: t read, W written, coupling structure and 1/D' read
-				timed_launch(c, "k_mk_apply", (double)B * nU * (8.0 + sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz,
-							 [&] {
-								 k_mk_apply<T><<<grid, sblock, 0, c.stream>>>(S, G, n_seg, t.get(), W.get(), live.get());
-							 });
-				// classical Gram-Schmidt, twice in FP64 and once in FP32 cycles (the
-				// second-pass coefficients c2 then stay zero)
-				const double dbytes = (double)B * nU * sizeof(T) * (j + 2);
-				timed_launch(c, "k_dots", dbytes, [&] {
-					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride, part.get(),
-																 live.get());
+				timed_launch(c, "k_w_apply", (double)cnt * (2.0 * sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz, [&] {
+					k_w_apply<T, T><<<gq, sblock, 0, c.stream>>>(G, nU, B, rt.get(), winvd.get(), wanc.get(), t.get(),
+																 W.get(), live.get());
 				});
-				k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c1.get(), m + 1, 0);
-				timed_launch(c, "k_orth_update", dbytes + (double)B * nU * sizeof(T), [&] {
-					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride, c1.get(),
-																	 m + 1, part.get(), live.get());
+				const double dbytes = (double)cnt * sizeof(T) * (j + 2);
+				timed_launch(c, "k_w_dots", dbytes, [&] {
+					k_w_dots<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, part.get(),
+														   live.get());
+				});
+				k_w_reduce<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nred, B, j + 1, c1.get(), m + 1,
+																   ct.get());
+				timed_launch(c, "k_w_orth", dbytes + (double)cnt * sizeof(T), [&] {
+					k_w_orth<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, ct.get(),
+														   part.get(), live.get());
 				});
 				if (sizeof(T) != 4)
 				{
-					k_dots<T, 64><<<grid, sblock, 0, c.stream>>>(nU, B, j + 1, W.get(), V.get(), vstride,
-																 part.get(), live.get());
-					k_reduce_parts<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nblk, j + 1, c2.get(), m + 1,
-																		  0);
-					k_orth_update<T><<<grid, sblock, 0, c.stream>>>(nU, j + 1, W.get(), V.get(), vstride,
-																	 c2.get(), m + 1, part.get(), live.get());
+					k_w_dots<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, part.get(),
+														   live.get());
+					k_w_reduce<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nred, B, j + 1, c2.get(), m + 1,
+																	   ct.get());
+					k_w_orth<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, ct.get(),
+														   part.get(), live.get());
 				}
-				k_reduce_parts<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nblk, 1, hn.get(), 1, 0);
+				k_w_reduce<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nred, B, 1, hn.get(), 1, nullptr);
 				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st, 0.5 * tol, sizeof(T) == 4 ? inner_rtol : 0.0);
-				k_scale<T><<<grid, sblock, 0, c.stream>>>(nU, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride,
+				k_w_scale<T><<<gq, sblock, 0, c.stream>>>(cnt, B, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride,
 														  live.get());
 				ESL_HIP(hipGetLastError());
 				++iters;
@@ -3295,14 +3631,14 @@ struct gmres_t
 			}
 			// x += P^{-1} (sum_i y_i V_i) in FP64
 			k_gm_solve_y<<<sb, 64, 0, c.stream>>>(B, m, k, st);
-			k_zero<double><<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get());
-			k_combine<T, double><<<grid, sblock, 0, c.stream>>>(nU, k, V.get(), vstride, c1.get(), m, r.get(),
-															   kb.get());
+			k_w_transpose_y<<<grid((int64_t)B * (m + 1)), sblock, 0, c.stream>>>(B, m, c1.get(), yt.get());
+			k_zero<double><<<gq, sblock, 0, c.stream>>>(cnt, r.get());
+			k_w_combine<T><<<gq, sblock, 0, c.stream>>>(cnt, B, k, V.get(), vstride, yt.get(), r.get(), kb.get());
 			precondition<double>(r.get(), act.get());
 			if (sizeof(T) == 4) // keep the correction in t until the next residual has been checked
-				k_axpy1_keep<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x, t.get());
+				k_axpy1_keep<<<gq, sblock, 0, c.stream>>>(cnt, r.get(), wx.get(), keep.get());
 			else
-				k_axpy1<<<ceil_div(vstride, sblock), sblock, 0, c.stream>>>(vstride, r.get(), x);
+				k_axpy1<<<gq, sblock, 0, c.stream>>>(cnt, r.get(), wx.get());
 			ESL_HIP(hipGetLastError());
 		}
 	}
@@ -3719,49 +4055,69 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	}
 	else
 	{
-		// device-wide solver: the iterate is x = invd * z inside
+		// device-wide solver on the set-minor layout, at most 1024 sets at a time
 		nbr_store_t local;
 		if (!prebuilt)
 			build_nbr(c, S, local);
 		const nbr_store_t& nb = prebuilt ? *prebuilt : local;
 		const bool ilu = opts.preconditioner != 1;
-		const int64_t cnt = (int64_t)B * nU;
-		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 0);
-		bool done = false;
-		if (opts.precision == 1)
+		const int nseg1 = std::max(n_seg, 1);
+		std::vector<double> res_all;
+		for (int b0 = 0; b0 < B; b0 += 1024)
 		{
-			gmres_t<float> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			done = gm.solve(rhs_p, z_p, tol, max_iters, opts.inner_rtol);
-			if (set_res)
-				*set_res = gm.h_res; // the residuals of its last check
-			kr.resid = gm.final_res;
-			kr.iters = gm.iters;
-			kr.cycles = gm.cycles;
-			if (!done)
-				kr.fp64_fallback = B;
-		}
-		if (!done)
-		{
-			// FP64 from the current iterate (also what an FP32 solve that stalled falls back to)
-			gmres_t<double> gm(c, S, nb, std::max(n_seg, 1), restart, ilu);
-			const bool done64 = gm.solve(rhs_p, z_p, tol, std::max(max_iters - kr.iters, 1));
-			if (set_res)
-				*set_res = gm.h_res;
-			kr.resid = gm.final_res;
-			kr.iters += gm.iters;
-			kr.cycles += gm.cycles;
-			if (!done64)
+			const int cb = std::min(1024, B - b0);
+			sys_t Sc = S;
+			Sc.B = cb;
+			Sc.rates = S.rates + (int64_t)b0 * S.rate_stride;
+			Sc.anchor = S.anchor + (int64_t)b0 * nseg1;
+			Sc.invd = S.invd + (int64_t)b0 * nU;
+			const double* rhs_c = rhs_p + (int64_t)b0 * nU;
+			double* z_c = z_p + (int64_t)b0 * nU;
+			bool done = false;
+			int it = 0, cy = 0;
+			std::vector<double> cres;
+			double cmax = 0.0;
+			if (opts.precision == 1)
 			{
-				if (throw_on_miss)
-					throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
-												   sci(kr.resid) + " after " + std::to_string(kr.iters) +
-												   " iterations");
-				kr.missed = B; // an upper bound: the caller reads every set's residual
+				gmres_w_t<float> gm(c, Sc, nb, nseg1, restart, ilu);
+				gm.enter(rhs_c, z_c);
+				done = gm.solve(tol, max_iters, opts.inner_rtol);
+				gm.leave(z_c);
+				cres = gm.h_res; // the residuals of its last check
+				cmax = gm.final_res;
+				it = gm.iters;
+				cy = gm.cycles;
+				if (!done)
+					kr.fp64_fallback += cb;
 			}
+			if (!done)
+			{
+				// FP64 from the current iterate (also what an FP32 solve that stalled falls back to)
+				gmres_w_t<double> gm(c, Sc, nb, nseg1, restart, ilu);
+				gm.enter(rhs_c, z_c);
+				const bool done64 = gm.solve(tol, std::max(max_iters - it, 1));
+				gm.leave(z_c);
+				cres = gm.h_res;
+				cmax = gm.final_res;
+				it += gm.iters;
+				cy += gm.cycles;
+				if (!done64)
+				{
+					if (throw_on_miss)
+						throw error(ESL_E_NUMERIC, "GMRES did not reach the tolerance: relative residual " +
+													   sci(cmax) + " after " + std::to_string(it) + " iterations");
+					kr.missed += cb; // an upper bound: the caller reads every set's residual
+				}
+			}
+			res_all.insert(res_all.end(), cres.begin(), cres.end());
+			kr.resid = std::max(kr.resid, cmax);
+			kr.iters = std::max(kr.iters, it);
+			kr.cycles = std::max(kr.cycles, cy);
+			kr.sets_iterated += cy > 0 ? cb : 0;
+			kr.iters_sum += (int64_t)it * cb; // the sets of a chunk iterate together
 		}
-		kr.sets_iterated = kr.cycles > 0 ? B : 0;
-		kr.iters_sum = (int64_t)kr.iters * B; // the sets iterate together
-		k_scale_invd<<<ceil_div(cnt, sblock), sblock, 0, c.stream>>>(cnt, S.invd, z_p, 1);
+		if (set_res)
+			*set_res = std::move(res_all);
 		ESL_HIP(hipGetLastError());
 	}
 	return kr;

@@ -62,3 +62,110 @@ def test_compat_header_solve(tmp_path):
     t, a, b, c = out[1].split()
     # rates.toy golden (cuda/test/test.cu:556-558): 1/2, 11/48, 13/48
     assert t == "3" and abs(float(a) - 0.5) < 1e-5 and abs(float(b) - 11 / 48) < 1e-5 and abs(float(c) - 13 / 48) < 1e-5
+
+
+# cuda/test/test.cu solver.toy (:67-130) and the .symb flows (:561-925), written
+# with the reference's class names and signatures over the compat header
+PROG_SOLVER = r"""
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+#include "esl_compat.hpp"
+template <typename V> void show(const char* name, const V& v) {
+    std::printf("%s", name);
+    for (auto x : v) std::printf(" %g", (double)x);
+    std::printf("\n");
+}
+int main(int argc, char** argv) {
+    compat::cu_context context;
+    compat::model_builder builder;
+    auto model = builder.construct_model(argv[1]);
+    compat::transition_rates r(model);
+    r.generate_uniform();
+    compat::transition_table table(context, model, r.rates);
+    table.construct_table();
+    compat::transition_graph g(context, table.rows, table.cols, table.indptr);
+    g.reorganize_vertices();
+    compat::initial_state st(model, {"A", "C", "D"}, {false, false, false}, 1.f);
+    compat::solver s(context, table, g, r, st);
+    s.solve();
+    show("order", g.reordered_vertices);
+    show("term_indptr", s.solution_term.indptr);
+    show("term_indices", s.solution_term.indices);
+    show("term_data", s.solution_term.data);
+    show("nonterm_indptr", s.solution_nonterm.indptr);
+    show("nonterm_indices", s.solution_nonterm.indices);
+    show("nonterm_data", s.solution_nonterm.data);
+    show("final", s.final_state);
+    std::ostringstream os;
+    s.print_final_state(model.nodes, os);
+    std::printf("printed %s", os.str().c_str());
+
+    compat::persistent_solution::serialize(argv[2], s, true);
+    auto persisted = compat::persistent_solution::deserialize(argv[2], true);
+    compat::solver same(context, persisted, r, st);
+    same.solve();
+    double d_same = 0;
+    for (size_t i = 0; i < s.final_state.size(); ++i) d_same += std::fabs(same.final_state[i] - s.final_state[i]);
+    compat::transition_rates r2(model);
+    r2.generate_uniform({}, {{"C", 2.0}});
+    std::printf("compatible %d same %d\n", (int)compat::persistent_solution::has_compatible_zero_rates(persisted, r2.rates),
+                (int)compat::persistent_solution::are_same(persisted, r2.rates));
+    compat::solver changed(context, persisted, r2, st);
+    changed.solve();
+    compat::transition_table table2(context, model, r2.rates);
+    table2.construct_table();
+    compat::transition_graph g2(context, table2.rows, table2.cols, table2.indptr);
+    g2.reorganize_vertices();
+    compat::solver fresh(context, table2, g2, r2, st);
+    fresh.solve();
+    double d_changed = 0;
+    for (size_t i = 0; i < s.final_state.size(); ++i) d_changed += std::fabs(changed.final_state[i] - fresh.final_state[i]);
+    std::printf("recompute %d %d l1 %g %g\n", (int)same.recompute_needed, (int)changed.recompute_needed, d_same, d_changed);
+    try { compat::transition_graph bad(context, table2.rows, table2.cols, table.indptr); std::printf("no error\n"); }
+    catch (std::exception&) { std::printf("mixed vectors rejected\n"); }
+    return 0;
+}
+"""
+
+
+def _build_solver(tmp_path):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    src = tmp_path / "s.cpp"
+    src.write_text(PROG_SOLVER)
+    exe = tmp_path / "s"
+    libdir = os.path.dirname(LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-lesl", f"-Wl,-rpath,{libdir}"], check=True)
+    return exe
+
+
+def test_compat_solver_body_compiles(tmp_path):
+    _build_solver(tmp_path)
+
+
+@pytest.mark.gpu
+def test_compat_solver_toy_like_reference_test(tmp_path):
+    from .helpers import golden
+
+    c = golden()["solver.toy"]
+    exe = _build_solver(tmp_path)
+    out = subprocess.run([str(exe), model_path("toy.bnet"), str(tmp_path / "toy.symb")], capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    rec = {ln.split()[0]: ln.split()[1:] for ln in out if ln and not ln.startswith("printed")}
+    ints = lambda k: [int(float(x)) for x in rec[k]]  # noqa: E731
+    assert ints("order") == c["reordered_vertices"]
+    assert ints("term_indptr") == c["term_indptr"] and ints("term_indices") == c["term_indices"]
+    assert [float(x) for x in rec["term_data"]] == pytest.approx(c["term_data"], abs=1e-12)
+    assert ints("nonterm_indptr") == c["nonterm_indptr"] and ints("nonterm_indices") == c["nonterm_indices"]
+    assert [float(x) for x in rec["nonterm_data"]] == pytest.approx(c["nonterm_data"], abs=1e-12)
+    fin = [float(x) for x in rec["final"]]
+    assert [i for i, v in enumerate(fin) if v != 0] == c["nonzero_indices"]
+    # print_final_state (solver.cu:336-372): toy's nodes are A, C, D, so states 2
+    # and 4 print as "C 0.5" and "D 0.5"
+    assert "printed C 0.5" in out and "D 0.5" in out
+    assert rec["compatible"] == ["1", "same", "0"]
+    r = rec["recompute"]
+    assert r[0] == "0" and r[1] == "1" and float(r[3]) <= 1e-12 and float(r[4]) <= 1e-12
+    assert "mixed vectors rejected" in out
