@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--autotune", type=int, default=1, help="pick the schedule with esl_plan_autotune first")
     ap.add_argument("--schedule", type=int, default=None,
                     help="pin a schedule instead of autotuning (0 per-set, 1 per-set global, 2 device-wide, "
-                         "3 per-set LDS with the basis in global memory)")
+                         "3 per-set LDS with the basis in global memory, 4/5 per-set at 3/2 workgroups per CU)")
     ap.add_argument("--workload", default="n20", choices=["n20", "n16", "uneven", "n22", "warm"],
                     help="n20 (default) is the headline metric; the others print their own JSON line")
     ap.add_argument("--epochs", type=int, default=30, help="training epochs of the warm workload")
@@ -136,7 +136,8 @@ def copy_calibration(L, ctx):
     print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), file=sys.stderr)
 
 
-SCHEDULE_NAMES = ["per-set LDS-resident", "per-set global", "device-wide", "per-set LDS, basis in memory"]
+SCHEDULE_NAMES = ["per-set LDS-resident", "per-set global", "device-wide", "per-set LDS, basis in memory",
+                  "per-set LDS-resident, 3 workgroups per CU", "per-set LDS-resident, 2 workgroups per CU"]
 
 
 def host_cpu():

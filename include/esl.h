@@ -253,9 +253,11 @@ esl_status esl_plan_gather_terminal(esl_ctx* ctx, const esl_plan* p, int n_sets,
 /* Autotuner: runs this batch under each candidate schedule -- 0 per-set solve with
  * the system in LDS (FP32 basis in registers where it fits), 1 per-set from global
  * memory, 2 device-wide, 3 per-set with the system in LDS and the basis in global
- * memory -- and restart length 16 / 24 / 32 / 64, keeps the fastest that meets
- * opts->tol in the plan, and returns it (choice[0] = schedule 0..3, choice[1] =
- * restart; *best_ms = its time per batch).
+ * memory, 4 and 5 as 0 with 3 and 2 workgroups per CU (more registers: the basis of
+ * systems up to 1024 unknowns then stays in registers) -- and restart length
+ * 16 / 24 / 32 / 64, keeps the fastest that meets opts->tol in the plan, and returns
+ * it (choice[0] = schedule 0..5, choice[1] = restart; *best_ms = its time per
+ * batch).
  * d_final receives the batch's results.  Later esl_plan_solve / esl_plan_solve_many
  * calls with opts->restart == 0 use the stored choice (esl_plan_query info[7] =
  * 1000 * schedule + restart, or -1 before tuning).  No reference counterpart. */
@@ -275,7 +277,7 @@ esl_status esl_plan_set_results(const esl_plan* p, int n_sets, double* residual,
 								int32_t* method);
 
 /* Pins the schedule esl_plan_autotune would store (e.g. one found earlier):
- * schedule 0..3 as above with the given restart length, or -1 to forget it. */
+ * schedule 0..5 as above with the given restart length, or -1 to forget it. */
 esl_status esl_plan_set_schedule(esl_plan* p, int schedule, int restart);
 
 /* Several plans (distinct networks, e.g. of uneven size) in one call: plan i solves

@@ -666,7 +666,7 @@ esl_status esl_plan_set_schedule(esl_plan* p, int schedule, int restart)
 {
 	return guarded(nullptr, [&] {
 		require(p != nullptr, ESL_E_ARG, "null plan");
-		require(schedule >= -1 && schedule <= 3, ESL_E_ARG, "schedule must be -1, 0, 1, 2 or 3");
+		require(schedule >= -1 && schedule <= 5, ESL_E_ARG, "schedule must be -1 or 0..5");
 		esl::plan_set_schedule(p->p, schedule, restart);
 	});
 }

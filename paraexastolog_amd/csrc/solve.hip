@@ -2166,6 +2166,25 @@ __global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(4))) voi
 	gmres_lds_body<float, 4, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
 }
 
+// The same kernel with 170 or 256 registers (3 or 2 workgroups per CU): the
+// basis of restart 24 (100 floats per thread) then stays in registers, and a
+// batch of more sets than slots refills the slots that fast sets free.
+template <int MR>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(3))) void k_gmres_priv_o3(
+	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
+	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+{
+	gmres_lds_body<float, 4, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+}
+
+template <int MR>
+__global__ __launch_bounds__(pblock) __attribute__((amdgpu_waves_per_eu(2))) void k_gmres_priv_o2(
+	sys_t S, nbr_t G, int n_seg, const double* __restrict__ rhs, double* __restrict__ zio, pgm_scratch_t sc, int m,
+	double tol, int max_iters, pgm_out_t out, lds_plan_t lp, double inner_rtol)
+{
+	gmres_lds_body<float, 4, MR>(S, G, n_seg, rhs, zio, sc, m, tol, max_iters, out, lp, inner_rtol);
+}
+
 // Anchor choice from the level-ordered packed records of the LDS-resident
 // solver: the damped Jacobi sweeps of k_anchor_sweep (p' = p/2 + inflow/(2D))
 // over every unknown of the set in LDS, then per segment the unknown of
@@ -3761,8 +3780,10 @@ struct per_set_scratch_t
 };
 
 // one launch: GMRES of every set, one workgroup per set (no host synchronisation)
+// occ: workgroups per CU the private-basis kernel of small systems aims for (4, 3 or 2)
 void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs, double* z,
-					const esl_solve_opts& o, const pss_view_t& sc, bool allow_lds = true, bool allow_reg = true)
+					const esl_solve_opts& o, const pss_view_t& sc, bool allow_lds = true, bool allow_reg = true,
+					int occ = 4)
 {
 	const int B = S.B;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
@@ -3781,11 +3802,14 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			const bool small = S.nU <= 4 * pblock;
 			if (lp.bytes > 48 * 1024)
 			{
-				const void* fns[10] = { (const void*)k_gmres_lds<float, 4>,  (const void*)k_gmres_lds<float, 8>,
+				const void* fns[16] = { (const void*)k_gmres_lds<float, 4>,  (const void*)k_gmres_lds<float, 8>,
 										(const void*)k_gmres_lds<double, 4>, (const void*)k_gmres_lds<double, 8>,
 										(const void*)k_gmres_priv<32>,		 (const void*)k_gmres_priv<24>,
 										(const void*)k_gmres_priv<16>,		 (const void*)k_gmres_priv8<32>,
-										(const void*)k_gmres_priv8<24>,		 (const void*)k_gmres_priv8<16> };
+										(const void*)k_gmres_priv8<24>,		 (const void*)k_gmres_priv8<16>,
+										(const void*)k_gmres_priv_o3<32>,	 (const void*)k_gmres_priv_o3<24>,
+										(const void*)k_gmres_priv_o3<16>,	 (const void*)k_gmres_priv_o2<32>,
+										(const void*)k_gmres_priv_o2<24>,	 (const void*)k_gmres_priv_o2<16> };
 				for (const void* f : fns)
 					ESL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lp.bytes));
 			}
@@ -3808,17 +3832,30 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 			{
 				// the FP32 basis of each thread's rows as a private array
 				pgm_scratch_t sp{ sc.r, sc.udiag, sc.t, sc.Vf };
-				timed_launch(c, "k_gmres_priv<float>", by, [&] {
-					if (m <= 16)
-						k_gmres_priv<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																			 o.max_iters, po, lp, o.inner_rtol);
-					else if (m <= 24)
-						k_gmres_priv<24><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																			 o.max_iters, po, lp, o.inner_rtol);
-					else
-						k_gmres_priv<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol,
-																			 o.max_iters, po, lp, o.inner_rtol);
-				});
+#define ESL_PRIV_LAUNCH(KERNEL)                                                                                \
+	do                                                                                                         \
+	{                                                                                                          \
+		if (m <= 16)                                                                                           \
+			KERNEL<16><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters, po, lp, \
+														  o.inner_rtol);                                      \
+		else if (m <= 24)                                                                                      \
+			KERNEL<24><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters, po, lp, \
+														  o.inner_rtol);                                      \
+		else                                                                                                   \
+			KERNEL<32><<<B, pblock, lp.bytes, c.stream>>>(S, G, nseg1, rhs, z, sp, m, o.tol, o.max_iters, po, lp, \
+														  o.inner_rtol);                                      \
+	} while (0)
+				timed_launch(c, occ == 3 ? "k_gmres_priv_o3<float>" : occ == 2 ? "k_gmres_priv_o2<float>"
+																		: "k_gmres_priv<float>",
+							 by, [&] {
+								 if (occ == 3)
+									 ESL_PRIV_LAUNCH(k_gmres_priv_o3);
+								 else if (occ == 2)
+									 ESL_PRIV_LAUNCH(k_gmres_priv_o2);
+								 else
+									 ESL_PRIV_LAUNCH(k_gmres_priv);
+							 });
+#undef ESL_PRIV_LAUNCH
 			}
 			else if (o.precision == 1)
 			{
@@ -3975,6 +4012,7 @@ struct plan_t
 	} tune;
 	bool allow_lds = true; // the schedule of the run being enqueued
 	bool allow_reg = true;
+	int occ = 4;		   // workgroups per CU of the private-basis kernel (schedules 4 and 5: 3 and 2)
 	set_results_t results; // per-set outcome of the last run (esl_plan_set_results)
 	double ms[3] = { 0, 0, 0 };
 	// captured numeric sequence and the arguments it was captured with
@@ -3989,12 +4027,13 @@ struct plan_t
 		int max_iters = 0;
 		int64_t epoch = 0;
 		bool allow_lds = true, allow_reg = true;
+		int occ = 4;
 		bool operator==(const gkey_t& o) const
 		{
 			return B == o.B && rates == o.rates && fin == o.fin && warm == o.warm && precision == o.precision &&
 				   preconditioner == o.preconditioner && m == o.m && tol == o.tol && inner_rtol == o.inner_rtol &&
 				   max_iters == o.max_iters && epoch == o.epoch && allow_lds == o.allow_lds &&
-				   allow_reg == o.allow_reg;
+				   allow_reg == o.allow_reg && occ == o.occ;
 		}
 	};
 	gkey_t gkey{};
@@ -4528,7 +4567,7 @@ void plan_enqueue(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_f
 	plan_enqueue_setup(c, P, B, d_rates, o);
 	if (P.nU)
 		launch_per_set(c, P.sys(B, d_rates), P.nbr, P.n_seg, P.rb.rhs, P.rb.z, o, P.rb.ps, P.allow_lds,
-					   P.allow_reg);
+					   P.allow_reg, P.occ);
 	if (fork)
 		ESL_HIP(hipStreamWaitEvent(c.stream, c.fill_ev[1], 0));
 	plan_enqueue_post(c, P, B, d_rates, d_final, !fork);
@@ -4764,6 +4803,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		o.restart = P.tune.m;
 	P.allow_lds = !(tuned && P.tune.path == 1);
 	P.allow_reg = !(tuned && P.tune.path == 3);
+	P.occ = tuned && P.tune.path == 4 ? 3 : tuned && P.tune.path == 5 ? 2 : 4;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
 	const bool per_set = P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B));
 	rep.terminals_reached = P.Tr;
@@ -4778,7 +4818,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		{
 			const plan_t::gkey_t key{ B,		   d_rates,	  d_final,	   o.warm_pi,  o.precision,
 									  o.preconditioner, m,		  o.tol,	   o.inner_rtol, o.max_iters,
-									  P.epoch,		   P.allow_lds, P.allow_reg };
+									  P.epoch,		   P.allow_lds, P.allow_reg, P.occ };
 			if (!P.gexec || !(P.gkey == key) || P.gprof != c.prof.on)
 			{
 				P.drop_graph();
@@ -4983,6 +5023,7 @@ void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double*
 		o.restart = P.tune.m;
 	const bool allow_lds = !(tuned && P.tune.path == 1);
 	const bool allow_reg = !(tuned && P.tune.path == 3);
+	const int occ = tuned && P.tune.path == 4 ? 3 : tuned && P.tune.path == 5 ? 2 : 4;
 	const int m = std::min(std::max(o.restart > 0 ? o.restart : 32, 2), pm_max);
 	o.restart = m;
 	const bool per_set = P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B));
@@ -5001,6 +5042,7 @@ void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double*
 	P.rb = carve_run_bufs(P, B, m, cv);
 	P.allow_lds = allow_lds;
 	P.allow_reg = allow_reg;
+	P.occ = occ;
 	rep.terminals_reached = P.Tr;
 	rep.unknowns = P.nU;
 	rep.reachable = -1;
@@ -5107,6 +5149,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		const bool tuned = o_in.restart == 0 && P.tune.set;
 		P.allow_lds = !(tuned && P.tune.path == 1);
 		P.allow_reg = !(tuned && P.tune.path == 3);
+		P.occ = tuned && P.tune.path == 4 ? 3 : tuned && P.tune.path == 5 ? 2 : 4;
 		po[i].restart = std::min(std::max(tuned ? P.tune.m : (o.restart > 0 ? o.restart : 32), 2), pm_max);
 		if (P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B[i])))
 			conc.push_back(i);
@@ -5153,7 +5196,8 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 			key.epochs.push_back(static_cast<plan_t*>(plans[i])->epoch);
 			key.ms.push_back(po[i].restart);
 			key.lds.push_back((static_cast<plan_t*>(plans[i])->allow_lds ? 1 : 0) +
-							  (static_cast<plan_t*>(plans[i])->allow_reg ? 2 : 0));
+							  (static_cast<plan_t*>(plans[i])->allow_reg ? 2 : 0) +
+							  4 * static_cast<plan_t*>(plans[i])->occ);
 		}
 		key.precision = o.precision;
 		key.preconditioner = o.preconditioner;
@@ -5273,7 +5317,8 @@ void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final,
 // Times candidate schedules of a plan on the caller's own batch and keeps the
 // fastest one that meets the tolerance: per-set LDS-resident / per-set from
 // global memory / device-wide / per-set LDS-resident with the basis in global
-// memory, restart length 16 / 24 / 32 / 64.  Each candidate runs once untimed
+// memory / per-set LDS-resident with the private basis at 3 or 2 workgroups per
+// CU, restart length 16 / 24 / 32 / 64.  Each candidate runs once untimed
 // (graph capture, buffer growth) and three times timed (best of three).  The
 // chosen schedule then runs once more, so d_final holds its results.  Later
 // solves use the choice when they ask for restart 0.
@@ -5300,6 +5345,11 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 				cands.push_back({ 1, m });
 			if (ilu && o.precision == 1 && P.nU <= lds_max_unknowns && m <= 32) // otherwise path 3 is path 0
 				cands.push_back({ 3, m });
+			if (ilu && o.precision == 1 && P.nU <= 4 * pblock && m <= 32) // the private-basis kernel at 3 / 2 WG per CU
+			{
+				cands.push_back({ 4, m });
+				cands.push_back({ 5, m });
+			}
 			cands.push_back({ 2, m });
 		}
 	const plan_t::tune_t saved = P.tune;
