@@ -421,6 +421,22 @@ def main():
     stats = profile_stats(L, ctx)
     check(L.esl_profile(ctx.handle, 0), ctx.handle)
     ms_per_step = 1000.0 * elapsed / a.steps
+    # per-set start / end of the last step's solve launch (device wall clock): the
+    # launch is bound by its slowest set's dependent chain, not by bandwidth
+    t_start, t_end = np.zeros(B), np.zeros(B)
+    latency = None
+    if L.esl_plan_set_times(ctx.handle, pl, B, t_start.ctypes.data_as(C.POINTER(C.c_double)),
+                            t_end.ctypes.data_as(C.POINTER(C.c_double))) == 0:
+        dur = t_end - t_start
+        span = float(t_end.max() - t_start.min())
+        latency = {"bound": "slowest set (dependent chain per set)", "launch_span_ms": span,
+                   "set_ms": {"mean": float(dur.mean()), "median": float(np.median(dur)),
+                              "p90": float(np.percentile(dur, 90)), "max": float(dur.max()),
+                              "min": float(dur.min())},
+                   "slot_busy_frac": float(dur.sum() / (B * span)) if span > 0 else None,
+                   "note": "slot_busy_frac = sum of set times / (sets x launch span): every set has its own "
+                           "workgroup slot (1024 slots at 4 workgroups per CU), so 1 - frac is the time "
+                           "slots idle waiting for the slowest set"}
     value = B * a.steps / elapsed
 
     # ---- accuracy: batched vs plain FP64 on the checked sets
@@ -531,6 +547,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch, "traffic": None},
         "cpu_baseline": cpu,
+        "latency_roof": latency,
     }
     tr = traffic_record(dom[0], f"N=20 seed {a.seed} sets {B}")
     if tr is not None:

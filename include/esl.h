@@ -276,6 +276,13 @@ esl_status esl_plan_autotune(esl_ctx* ctx, esl_plan* p, int n_sets, const double
 esl_status esl_plan_set_results(const esl_plan* p, int n_sets, double* residual, int32_t* iterations,
 								int32_t* method);
 
+/* When each set's workgroup of the last per-set launch on p started and ended,
+ * in ms after the earliest start (device wall clock): the spread of per-set
+ * solve times and how long the launch waited for its slowest set.  ESL_E_STATE
+ * when that solve ran device-wide or with another n_sets.  Any output may be
+ * NULL.  No reference counterpart. */
+esl_status esl_plan_set_times(esl_ctx* ctx, const esl_plan* p, int n_sets, double* start_ms, double* end_ms);
+
 /* Pins the schedule esl_plan_autotune would store (e.g. one found earlier):
  * schedule 0..5 as above with the given restart length, or -1 to forget it. */
 esl_status esl_plan_set_schedule(esl_plan* p, int schedule, int restart);

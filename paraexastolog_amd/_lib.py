@@ -115,6 +115,7 @@ SIGNATURES = [
     ("esl_plan_gather_terminal", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_plan_set_schedule", C.c_int, [_VP, C.c_int, C.c_int]),
     ("esl_plan_set_results", C.c_int, [_VP, C.c_int, _P(C.c_double), _P(C.c_int32), _P(C.c_int32)]),
+    ("esl_plan_set_times", C.c_int, [_VP, _VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
     ("esl_plan_autotune", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(SolveOpts), _P(C.c_int32), _P(C.c_double)]),
     ("esl_plan_solve_many", C.c_int, [_VP, C.c_int, _P(_VP), _P(C.c_int), _P(_VP), _P(_VP), _P(SolveOpts),
                                       _P(SolveReport)]),

@@ -125,6 +125,9 @@ esl_status esl_ctx_create(int device, esl_ctx** out)
 		ctx->c.flags.ensure(64);
 		ESL_HIP(hipHostMalloc((void**)&ctx->c.h_flags, 64 * sizeof(int32_t), hipHostMallocDefault));
 		esl::noalloc_init(ctx->c);
+		int wclk = 0;
+		if (hipDeviceGetAttribute(&wclk, hipDeviceAttributeWallClockRate, device) == hipSuccess && wclk > 0)
+			ctx->c.wall_khz = wclk;
 		*out = ctx.release();
 	});
 }
@@ -659,6 +662,15 @@ esl_status esl_plan_set_results(const esl_plan* p, int n_sets, double* residual,
 		require(p != nullptr, ESL_E_ARG, "null plan");
 		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
 		esl::plan_set_results(p->p, n_sets, residual, iterations, method);
+	});
+}
+
+esl_status esl_plan_set_times(esl_ctx* ctx, const esl_plan* p, int n_sets, double* start_ms, double* end_ms)
+{
+	return guarded(ctx, [&] {
+		require(ctx && p, ESL_E_ARG, "null argument");
+		require(n_sets >= 1, ESL_E_ARG, "n_sets must be >= 1");
+		esl::plan_set_times(p->p, n_sets, ctx->c.wall_khz, start_ms, end_ms);
 	});
 }
 

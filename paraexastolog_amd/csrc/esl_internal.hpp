@@ -176,6 +176,7 @@ struct ctx_t
 	void (*multi_free)(void*) = nullptr;
 	// device address of the library's static scratch array on this device (noalloc_init)
 	void* noalloc_scratch = nullptr;
+	double wall_khz = 100000.0; // rate of the device wall clock (wall_clock64) in kHz
 };
 
 // size of the static device scratch behind esl_plan_solve_noalloc, and the
@@ -300,6 +301,7 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final, double* d_out);
 void plan_set_schedule(void* p, int path, int restart);
 void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* method);
+void plan_set_times(const void* p, int B, double wall_khz, double* start_ms, double* end_ms);
 // the plan solve whose buffers are slices of the static scratch (no allocation)
 void noalloc_init(ctx_t& c); // once per device, with that device current
 void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,

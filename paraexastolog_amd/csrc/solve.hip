@@ -1032,6 +1032,7 @@ struct pgm_out_t
 	int32_t* cycles;  // [B]
 	double* res;	  // [B] final relative FP64 residual
 	int32_t* stalled; // [B] 1 when FP32 cycles stopped reducing the FP64 residual
+	uint64_t* tspan;  // [B][2] device wall clock when the set's workgroup started and ended
 };
 
 struct pgm_scratch_t
@@ -1134,6 +1135,7 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 	__shared__ double s_rates[64];
 
 	const int b = blockIdx.x;
+	const uint64_t t_begin = wall_clock64();
 	const int64_t nU = S.nU;
 	const int64_t off = (int64_t)b * nU;
 	const double* __restrict__ rb = rhs + off;
@@ -1363,6 +1365,8 @@ __global__ __launch_bounds__(pblock) void k_gmres_set(sys_t S, nbr_t G, int n_se
 		out.cycles[b] = cycles;
 		out.res[b] = rel;
 		out.stalled[b] = stalled;
+		out.tspan[2 * b] = t_begin;
+		out.tspan[2 * b + 1] = wall_clock64();
 	}
 }
 
@@ -1764,6 +1768,7 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 	__shared__ int s_loff[34];
 
 	const int b = blockIdx.x;
+	const uint64_t t_begin = wall_clock64();
 	const int tid = threadIdx.x, lane = tid & (wave - 1), wv = tid / wave;
 	const int nU = (int)S.nU;
 	const int nl = G.L.n;
@@ -2130,6 +2135,8 @@ __device__ __forceinline__ void gmres_lds_body(const sys_t& S, const nbr_t& G, i
 		out.cycles[b] = cycles;
 		out.res[b] = rel;
 		out.stalled[b] = stalled;
+		out.tspan[2 * b] = t_begin;
+		out.tspan[2 * b + 1] = wall_clock64();
 	}
 }
 
@@ -3700,6 +3707,7 @@ struct pss_view_t
 {
 	double *r = nullptr, *udiag = nullptr, *res = nullptr, *t = nullptr;
 	int32_t *its = nullptr, *cyc = nullptr, *stl = nullptr;
+	uint64_t* tspan = nullptr; // [B][2] start / end wall clock of each set's workgroup
 	float* Vf = nullptr;  // FP32 basis (precision 1)
 	double* Vd = nullptr; // FP64 basis (precision 0, and the FP64 finish of precision 1)
 };
@@ -3711,6 +3719,7 @@ struct set_results_t
 {
 	std::vector<double> res;
 	std::vector<int32_t> its, method;
+	std::vector<uint64_t> tspan; // per-set start / end wall clock of the last per-set launch (empty otherwise)
 };
 
 std::string sci(double v)
@@ -3755,6 +3764,7 @@ struct per_set_scratch_t
 {
 	dbuf<double> r, udiag, res, t;
 	dbuf<int32_t> its, cyc, stl;
+	dbuf<uint64_t> tspan;
 	dbuf<float> Vf;
 	dbuf<double> Vd;
 	void reserve(int B, int64_t nU, int m, int precision)
@@ -3766,6 +3776,7 @@ struct per_set_scratch_t
 		its.ensure(B);
 		cyc.ensure(B);
 		stl.ensure(B);
+		tspan.ensure((size_t)2 * B);
 		t.ensure((size_t)B * nU);
 		if (precision == 1)
 			Vf.ensure((size_t)B * (m + 1) * nU);
@@ -3775,7 +3786,7 @@ struct per_set_scratch_t
 	pss_view_t view() const
 	{
 		return pss_view_t{ r.get(), udiag.get(), res.get(), t.get(), its.get(), cyc.get(), stl.get(),
-						   Vf.get(), Vd.get() };
+						   tspan.get(), Vf.get(), Vd.get() };
 	}
 };
 
@@ -3790,7 +3801,7 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 	const int nseg1 = std::max(n_seg, 1);
 	const bool ilu = o.preconditioner != 1; // ILU(0) unless Jacobi is asked for
 	const nbr_t G = nb.view();
-	pgm_out_t po{ sc.its, sc.cyc, sc.res, sc.stl };
+	pgm_out_t po{ sc.its, sc.cyc, sc.res, sc.stl, sc.tspan };
 	const double by = gmres_set_bytes(S, nb.nnz);
 	if (allow_lds && ilu && nb.packed)
 	{
@@ -3929,6 +3940,7 @@ krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, 
 	set_results_t sr;
 	sr.method.assign(B, 0);
 	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its);
+	read_host(c, sr.tspan, sc.tspan, (size_t)2 * B);
 	if (o.precision == 1 && kr.missed > 0)
 	{
 		o.precision = 0;
@@ -4584,7 +4596,7 @@ void plan_reserve(plan_t& P, int B, int m, int precision)
 										 P.rhs.get(),	 P.z.get(),		  P.y.get(),	  P.segsum.get(),
 										 P.scr.r.get(),	  P.scr.udiag.get(), P.scr.res.get(),
 										 P.scr.t.get(), P.scr.its.get(), P.scr.cyc.get(), P.scr.stl.get(),
-										 P.scr.Vf.get(), P.scr.Vd.get() };
+										 P.scr.tspan.get(), P.scr.Vf.get(), P.scr.Vd.get() };
 	};
 	const auto before = ptrs();
 	plan_reserve_(P, B, m, precision);
@@ -4992,6 +5004,7 @@ plan_t::run_bufs_t carve_run_bufs(const plan_t& P, int B, int m, carve_t& cv)
 	rb.ps.its = cv.take<int32_t>(B);
 	rb.ps.cyc = cv.take<int32_t>(B);
 	rb.ps.stl = cv.take<int32_t>(B);
+	rb.ps.tspan = cv.take<uint64_t>((size_t)2 * B);
 	rb.ps.Vd = cv.take<double>((size_t)B * (m + 1) * nU);
 	rb.ps.Vf = reinterpret_cast<float*>(rb.ps.Vd);
 	return rb;
@@ -5428,6 +5441,26 @@ void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* 
 			its[b] = P.results.its[b];
 		if (method)
 			method[b] = P.results.method[b];
+	}
+}
+
+// per-set start / end of the last per-set launch in ms after its first start
+void plan_set_times(const void* p, int B, double wall_khz, double* start_ms, double* end_ms)
+{
+	const plan_t& P = *static_cast<const plan_t*>(p);
+	const auto& t = P.results.tspan;
+	if (t.size() != (size_t)2 * B)
+		throw error(ESL_E_STATE, "the last solve of this plan recorded no per-set times for " + std::to_string(B) +
+									 " sets (it ran device-wide or with another batch size)");
+	uint64_t t0 = UINT64_MAX;
+	for (int b = 0; b < B; ++b)
+		t0 = std::min(t0, t[2 * b]);
+	for (int b = 0; b < B; ++b)
+	{
+		if (start_ms)
+			start_ms[b] = (double)(t[2 * b] - t0) / wall_khz;
+		if (end_ms)
+			end_ms[b] = (double)(t[2 * b + 1] - t0) / wall_khz;
 	}
 }
 

@@ -496,3 +496,25 @@ def test_results_are_bitwise_reproducible(ctx, case):
         plan.close()
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+def test_per_set_times_of_a_launch(ctx):
+    # esl_plan_set_times: every set of the last per-set launch has a start and
+    # an end, end >= start, all within one launch span
+    import ctypes as C
+
+    from paraexastolog_amd._lib import EslError, check, lib
+
+    B = 16
+    net, rates, m, t, g, x0 = setup_net(ctx, "c1", 0, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12)
+    s0, s1 = np.zeros(B), np.zeros(B)
+    check(lib().esl_plan_set_times(ctx.handle, plan.handle, B, s0.ctypes.data_as(C.POINTER(C.c_double)),
+                                   s1.ctypes.data_as(C.POINTER(C.c_double))), ctx.handle)
+    assert s0.min() == 0.0 and (s1 >= s0).all() and 0 < s1.max() < 1000.0
+    assert lib().esl_plan_set_times(ctx.handle, plan.handle, B + 1, None, None) != 0  # another batch size
