@@ -61,6 +61,53 @@ __global__ __launch_bounds__(table_block) void k_flip_mask(int N, int64_t n, con
 		block_deg[blockIdx.x] = tot;
 }
 
+// The rules of a network whose every node reads at most 6 nodes, by value in
+// the kernel arguments (uniform, so the loads are scalar): per node its input
+// indices packed 5 bits each, its truth table as one 64-bit word and its
+// input count.  k_flip_mask_small evaluates a rule with a few shifts and one
+// table lookup in registers, where k_flip_mask reads the rule, then its inputs,
+// then the table word from memory for every node of every state.
+constexpr int small_rule_inputs = 6;
+struct rules_small_t
+{
+	uint64_t tab[32];
+	uint32_t pin[32];
+	uint8_t k[32];
+};
+
+__global__ __launch_bounds__(table_block) void k_flip_mask_small(int N, int64_t n, rules_small_t R, uint32_t up_ok,
+																 uint32_t down_ok, uint32_t* __restrict__ mask,
+																 int32_t* __restrict__ block_deg)
+{
+	__shared__ int32_t s_red[table_block / wave + 1];
+	const int64_t s = (int64_t)blockIdx.x * table_block + threadIdx.x;
+	int deg = 0;
+	if (s < n)
+	{
+		const uint32_t st = (uint32_t)s;
+		uint32_t m = 0;
+		for (int i = 0; i < N; ++i)
+		{
+			const uint32_t p = R.pin[i];
+			const int k = R.k[i];
+			uint32_t idx = 0;
+#pragma unroll
+			for (int q = 0; q < small_rule_inputs; ++q)
+				if (q < k)
+					idx |= ((st >> ((p >> (5 * q)) & 31u)) & 1u) << q;
+			const uint32_t f = (uint32_t)(R.tab[i] >> idx) & 1u;
+			const uint32_t cur = (st >> i) & 1u;
+			const uint32_t allowed = cur ? ((down_ok >> i) & 1u) : ((up_ok >> i) & 1u);
+			m |= ((f ^ cur) & allowed) << i;
+		}
+		mask[s] = m;
+		deg = __popc(m);
+	}
+	const int tot = block_sum<table_block>(deg, s_red);
+	if (threadIdx.x == 0)
+		block_deg[blockIdx.x] = tot;
+}
+
 // indptr[s] = block offset + in-block exclusive prefix of the degrees; rows and
 // cols are staged in LDS and written as the block's contiguous output run.
 __global__ __launch_bounds__(table_block) void k_table_fill(int N, int64_t n, const uint32_t* __restrict__ mask,
@@ -230,13 +277,35 @@ void table_build(ctx_t& c, graph_t& g, const model_t& m)
 	dbuf<int32_t> block_deg;
 	block_deg.ensure(nb + 1);
 
+	bool small = N <= 32;
+	for (int i = 0; i < N && small; ++i)
+		small = (int)m.rules[i].inputs.size() <= small_rule_inputs;
 	ESL_HIP(hipEventRecord(c.ev[0], c.stream));
 	// algorithmic bytes: the 4-byte mask written per state
-	timed_launch(c, "k_flip_mask", 4.0 * n, [&] {
-		k_flip_mask<<<nb, table_block, 0, c.stream>>>(N, n, g.rules.get(), g.rule_inputs.get(), g.rule_tables.get(),
-													  (int)tabs.size(), g.up_ok, g.down_ok, g.mask.get(),
-													  block_deg.get());
-	});
+	if (small)
+	{
+		rules_small_t R{};
+		for (int i = 0; i < N; ++i)
+		{
+			const rule_t& r = m.rules[i];
+			R.k[i] = (uint8_t)r.inputs.size();
+			for (size_t q = 0; q < r.inputs.size(); ++q)
+				R.pin[i] |= (uint32_t)r.inputs[q] << (5 * q);
+			for (size_t t = 0; t < r.table.size(); ++t)
+				if (r.table[t])
+					R.tab[i] |= uint64_t(1) << t;
+		}
+		timed_launch(c, "k_flip_mask", 4.0 * n, [&] {
+			k_flip_mask_small<<<nb, table_block, 0, c.stream>>>(N, n, R, g.up_ok, g.down_ok, g.mask.get(),
+																block_deg.get());
+		});
+	}
+	else
+		timed_launch(c, "k_flip_mask", 4.0 * n, [&] {
+			k_flip_mask<<<nb, table_block, 0, c.stream>>>(N, n, g.rules.get(), g.rule_inputs.get(),
+														  g.rule_tables.get(), (int)tabs.size(), g.up_ok, g.down_ok,
+														  g.mask.get(), block_deg.get());
+		});
 	k_scan_single<<<1, 1024, 0, c.stream>>>(block_deg.get(), nb);
 	ESL_HIP(hipGetLastError());
 	int32_t total = 0;
