@@ -794,12 +794,46 @@ def workload_n22(a, ctx):
         "sums": [float(np.abs(x64.sum(axis=1) - 1).max()), float(np.abs(x32.sum(axis=1) - 1).max())]}))
 
 
+def surrogate_gemm_rate(L, ctx, sur, X, T, steps=40, batch=512):
+    """Device time of the surrogate's BF16 GEMM kernels over `steps` training
+    steps (one esl_mlp_train_steps call at learning rate 0, so the trained
+    network is left as it is) on minibatches of 512 rows, timed per launch with
+    HIP events: the algorithmic 2 M N K flops of every product over the summed
+    kernel time, against the 2.5 PFLOP/s dense BF16 roof.  The products are
+    512 x 256 x 32 up to 512 x 2048 x 256 (and their transposes for the
+    gradients); event timing adds a few microseconds to each launch, so the
+    kernel-trace profile gives the tighter figure."""
+    from paraexastolog_amd._lib import check
+    from paraexastolog_amd.surrogate import DeviceArray
+
+    X = np.ascontiguousarray(X[:steps * batch], dtype=np.float32)
+    T = np.ascontiguousarray(T[:steps * batch], dtype=np.float32)
+    dX, dT = DeviceArray.from_host(ctx, X), DeviceArray.from_host(ctx, T)
+    check(L.esl_profile(ctx.handle, 1), ctx.handle)
+    t = time.perf_counter()
+    sur.train_steps_device(steps, batch, dX.ptr, dT.ptr, 0.0, 0.0)
+    wall = time.perf_counter() - t
+    stats = profile_stats(L, ctx)
+    check(L.esl_profile(ctx.handle, 0), ctx.handle)
+    g = [x for x in stats if x[0] == "k_gemm_bf16"]
+    r = [x for x in stats if x[0] == "k_gemm_reduce"]
+    launches = sum(x[1] for x in g)
+    ms = sum(x[2] for x in g)
+    flops = sum(x[3] for x in g)
+    tf = flops / (ms * 1e-3) / 1e12 if ms > 0 else None
+    return {"launches": launches, "ms": ms, "tflops": tf, "peak_tflops": 2500.0,
+            "frac": tf / 2500.0 if tf else None, "reduce_ms": sum(x[2] for x in r),
+            "step_ms_profiled": 1e3 * wall / steps,
+            "all_kernels_ms_per_step": sum(x[2] for x in stats) / steps}
+
+
 def workload_warm(a, ctx):
     """The learned warm start on the N=16 topology (seed 11, 2048 terminal
     states): 50k pipeline-generated training solves, the MLP trained on the GPU,
     4096 held-out rate sets solved cold and warm (prediction included in the
     warm timing)."""
     from paraexastolog_amd import synth
+    from paraexastolog_amd._lib import lib
     from paraexastolog_amd.surrogate import DeviceArray, Surrogate, log_rates, terminal_targets, train
 
     seed = 11 if a.seed is None else a.seed
@@ -815,6 +849,7 @@ def workload_warm(a, ctx):
     t = time.perf_counter()
     hist = train(sur, X[:n_train], T[:n_train], epochs=a.epochs, batch=512, lr=1e-2)
     train_s = time.perf_counter() - t
+    gemm = surrogate_gemm_rate(lib(), ctx, sur, X[:n_train], T[:n_train])
     Tt = T[n_train:]
     ent = float(-(np.where(Tt > 0, Tt * np.log(np.maximum(Tt, 1e-300)), 0.0)).sum(axis=1).mean())
     dX = DeviceArray.from_host(ctx, X[n_train:])
@@ -842,7 +877,8 @@ def workload_warm(a, ctx):
         "data_generation": {"solves": n_train + n_test, "s": gen_s, "solves_per_s": (n_train + n_test) / gen_s},
         "training": {"epochs": a.epochs, "steps": a.epochs * (n_train // 512), "s": train_s,
                      "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "held_out_target_entropy": ent,
-                     "tflops": 6.0 * 512 * n_par * a.epochs * (n_train // 512) / train_s / 1e12},
+                     "tflops": 6.0 * 512 * n_par * a.epochs * (n_train // 512) / train_s / 1e12,
+                     "gemm_kernel": gemm},
         "surrogate_error_l1": {"median": float(np.median(err)), "p90": float(np.percentile(err, 90)),
                                "max": float(err.max()), "constant_predictor_median": float(np.median(base))},
         "cold": {"sets_per_s": n_test / cold_s, "mean_iterations": cold["iterations_sum"] / n_test},

@@ -43,6 +43,12 @@ esl_status esl_mlp_gradients(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the mean cross-entropy */
 esl_status esl_mlp_train_step(esl_ctx* ctx, esl_mlp* m, int rows, const float* d_x, const float* d_target, float lr,
 							  float* loss_out);
+/* `steps` Adam steps on consecutive minibatches of `rows` rows (step j reads rows
+ * [j rows, (j + 1) rows) of d_x and d_target), the learning rate geometric from
+ * lr_first to lr_last; enqueued back to back with one host synchronisation at
+ * the end.  mean_loss_out (may be NULL): the mean loss over all the steps. */
+esl_status esl_mlp_train_steps(esl_ctx* ctx, esl_mlp* m, int steps, int rows, const float* d_x,
+							   const float* d_target, float lr_first, float lr_last, float* mean_loss_out);
 
 #ifdef __cplusplus
 }

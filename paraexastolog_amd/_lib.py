@@ -128,6 +128,7 @@ SIGNATURES = [
     ("esl_mlp_predict_warm", C.c_int, [_VP, _VP, C.c_int, _VP, _VP]),
     ("esl_mlp_gradients", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, _P(C.c_float), _P(C.c_float)]),
     ("esl_mlp_train_step", C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.c_float, _P(C.c_float)]),
+    ("esl_mlp_train_steps", C.c_int, [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_float, C.c_float, _P(C.c_float)]),
     ("esl_profile_stat", C.c_int, [_VP, C.c_int, C.c_char_p, C.c_int, _P(C.c_int64), _P(C.c_double),
                                    _P(C.c_double)]),
 ]

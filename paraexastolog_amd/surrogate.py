@@ -105,6 +105,14 @@ class Surrogate:
               self.ctx.handle)
         return float(loss.value)
 
+    def train_steps_device(self, steps, rows, d_x, d_t, lr_first, lr_last):
+        """`steps` Adam steps on consecutive minibatches of `rows` rows, one host
+        synchronisation (esl_mlp_train_steps); returns the mean loss."""
+        loss = C.c_float()
+        check(lib().esl_mlp_train_steps(self.ctx.handle, self.handle, steps, rows, d_x, d_t, C.c_float(lr_first),
+                                        C.c_float(lr_last), C.byref(loss)), self.ctx.handle)
+        return float(loss.value)
+
     def __del__(self):
         try:
             if self.handle:
@@ -155,10 +163,9 @@ def train(sur, X, T, epochs, batch=512, lr=2e-3, lr_final=1e-4):
     hist = []
     k = 0
     for _ in range(epochs):
-        acc = 0.0
-        for s in range(steps):
-            cur = lr * (lr_final / lr) ** (k / total)
-            acc += sur.train_step_device(batch, dX.offset(s * batch * n_in), dT.offset(s * batch * n_out), cur)
-            k += 1
-        hist.append(acc / steps)
+        # one call per epoch: the steps are enqueued without host round trips
+        first = lr * (lr_final / lr) ** (k / total)
+        last = lr * (lr_final / lr) ** ((k + steps - 1) / total)
+        hist.append(sur.train_steps_device(steps, batch, dX.ptr, dT.ptr, first, last))
+        k += steps
     return hist
