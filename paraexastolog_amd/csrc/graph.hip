@@ -36,23 +36,52 @@ namespace esl {
 
 constexpr int gblock = 256;
 
+// Moves along the high bits 6 .. N-1 read the neighbouring sub-cubes.  The
+// kernels below take them in a fully unrolled loop of HMAX steps, step j on
+// bit min(6 + j, N - 1) and counted only while 6 + j < N, so all of a lane's
+// neighbour loads are in flight before the first is used (a loop bounded by
+// the run-time N waits for each load in turn: one memory latency per bit).
+// HMAX is the smallest of 8, 16, 26 that covers N - 6 (0 below N = 7).
+__device__ __forceinline__ int high_bit(int j, int N) { return 6 + j < N ? 6 + j : N - 1; }
+
+// Fixpoint passes are enqueued in groups without waiting for the host; a
+// pass whose predecessor changed nothing (prev[0] == 0) returns at once.
+__device__ __forceinline__ bool converged(const int32_t* prev) { return prev && prev[0] == 0; }
+
 // ---- trimming: a vertex without an active in- or out-neighbour is a singleton SCC
+template <int HMAX>
 __global__ __launch_bounds__(gblock) void k_trim(int N, int64_t n, const uint32_t* __restrict__ mask,
-												 int32_t* __restrict__ scc, int32_t* __restrict__ changed)
+												 int32_t* __restrict__ scc, int32_t* __restrict__ changed,
+												 const int32_t* __restrict__ prev)
 {
+	if (converged(prev))
+		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const uint32_t mv = valid ? mask[v] : 0u;
 	bool active = valid && scc[v] < 0;
 	bool gin = false, gout = false;
-	if (active)
-		for (int i = 6; i < N; ++i)
+	if (HMAX > 0 && __any(active))
+	{
+		int32_t su[HMAX > 0 ? HMAX : 1];
+		uint32_t mu[HMAX > 0 ? HMAX : 1];
+		const int64_t vv = valid ? v : 0; // lanes past n read vertex 0's neighbours, unused
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
 		{
-			const int64_t u = v ^ (int64_t(1) << i);
-			const bool ua = scc[u] < 0;
-			gin |= ua && ((mask[u] >> i) & 1u);
+			const int64_t u = vv ^ (int64_t(1) << high_bit(j, N));
+			su[j] = scc[u];
+			mu[j] = mask[u];
+		}
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
+		{
+			const int i = 6 + j;
+			const bool ua = i < N && su[j] < 0;
+			gin |= ua && ((mu[j] >> i) & 1u);
 			gout |= ua && ((mv >> i) & 1u);
 		}
+	}
 	const int lim = N < 6 ? N : 6;
 	bool trimmed = false;
 	for (;;)
@@ -93,26 +122,40 @@ __global__ __launch_bounds__(gblock) void k_color_init(int64_t n, const int32_t*
 }
 
 // ---- min-colour propagation along edges (u -> v: col[v] = min(col[v], col[u]))
+template <int HMAX>
 __global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32_t* __restrict__ mask,
 												  const int32_t* __restrict__ scc, int32_t* __restrict__ col,
-												  int32_t* __restrict__ changed)
+												  int32_t* __restrict__ changed, const int32_t* __restrict__ prev)
 {
+	if (converged(prev))
+		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const bool active = valid && scc[v] < 0;
 	const uint32_t mv = valid ? mask[v] : 0u;
 	const int32_t c0 = active ? col[v] : INT32_MAX;
 	int32_t c = c0;
-	if (active)
-		for (int i = 6; i < N; ++i)
+	if (HMAX > 0 && __any(active))
+	{
+		uint32_t mu[HMAX > 0 ? HMAX : 1];
+		int32_t su[HMAX > 0 ? HMAX : 1], cu[HMAX > 0 ? HMAX : 1];
+		const int64_t vv = valid ? v : 0; // lanes past n read vertex 0's neighbours, unused
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
 		{
-			const int64_t u = v ^ (int64_t(1) << i);
-			if (((mask[u] >> i) & 1u) && scc[u] < 0)
-			{
-				const int32_t cu = col[u];
-				c = cu < c ? cu : c;
-			}
+			const int64_t u = vv ^ (int64_t(1) << high_bit(j, N));
+			mu[j] = mask[u];
+			su[j] = scc[u];
+			cu[j] = col[u];
 		}
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
+		{
+			const int i = 6 + j;
+			if (active && i < N && ((mu[j] >> i) & 1u) && su[j] < 0)
+				c = cu[j] < c ? cu[j] : c;
+		}
+	}
 	const int lim = N < 6 ? N : 6;
 	for (;;)
 	{
@@ -138,10 +181,14 @@ __global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32
 
 // ---- backward closure inside a colour class: v joins the SCC of its colour's
 // root if it has an edge into a member with the same colour.
+template <int HMAX>
 __global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32_t* __restrict__ mask,
 												  const int32_t* __restrict__ scc, const int32_t* __restrict__ col,
-												  uint8_t* __restrict__ inscc, int32_t* __restrict__ changed)
+												  uint8_t* __restrict__ inscc, int32_t* __restrict__ changed,
+												  const int32_t* __restrict__ prev)
 {
+	if (converged(prev))
+		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const bool active = valid && scc[v] < 0;
@@ -149,16 +196,29 @@ __global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32
 	const int32_t c = active ? col[v] : -1;
 	const bool in0 = active && inscc[v];
 	bool in = in0 || (active && c == (int32_t)v); // the colour's root is in its own SCC
-	if (active && !in)
-		for (int i = 6; i < N; ++i)
+	if (HMAX > 0 && __any(active && !in))
+	{
+		uint8_t iw[HMAX > 0 ? HMAX : 1];
+		int32_t cw[HMAX > 0 ? HMAX : 1], sw[HMAX > 0 ? HMAX : 1];
+		const int64_t vv = valid ? v : 0; // lanes past n read vertex 0's neighbours, unused
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
 		{
-			const int64_t w = v ^ (int64_t(1) << i);
-			if (((mv >> i) & 1u) && inscc[w] && col[w] == c && scc[w] < 0)
-			{
-				in = true;
-				break;
-			}
+			const int64_t w = vv ^ (int64_t(1) << high_bit(j, N));
+			iw[j] = inscc[w];
+			cw[j] = col[w];
+			sw[j] = scc[w];
 		}
+		bool hit = false;
+#pragma unroll
+		for (int j = 0; j < HMAX; ++j)
+		{
+			const int i = 6 + j;
+			hit |= i < N && ((mv >> i) & 1u) && iw[j] && cw[j] == c && sw[j] < 0;
+		}
+		if (active && hit)
+			in = true;
+	}
 	const int lim = N < 6 ? N : 6;
 	for (;;)
 	{
@@ -200,37 +260,48 @@ __global__ __launch_bounds__(gblock) void k_assign(int64_t n, int32_t* __restric
 		atomicAdd(remaining, cnt);
 }
 
-// ---- heights
+// ---- heights.  Level `cur` is enqueued in groups like the fixpoint passes;
+// once every root is labelled (total[0] == S) the kernels return at once.
+template <int HMAX>
 __global__ __launch_bounds__(gblock) void k_height_check(int N, int64_t n, const uint32_t* __restrict__ mask,
 														 const int32_t* __restrict__ scc,
 														 const int32_t* __restrict__ h, int32_t* __restrict__ blocked,
-														 int32_t cur)
+														 int32_t cur, const int32_t* __restrict__ total, int32_t S)
 {
+	if (total[0] >= S)
+		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	if (v >= n)
 		return;
 	const int32_t r = scc[v];
 	if (h[r] != 0)
 		return;
-	uint32_t rest = mask[v];
-	while (rest)
-	{
-		const int i = __ffs(rest) - 1;
-		rest &= rest - 1;
-		const int32_t rw = scc[v ^ (int64_t(1) << i)];
-		if (rw != r && h[rw] == 0)
-		{
-			blocked[r] = cur;
-			return;
-		}
-	}
+	const uint32_t mv = mask[v];
+	constexpr int B = HMAX + 6; // bits 0 .. N-1, clamped like the high bits
+	int32_t rw[B];
+#pragma unroll
+	for (int j = 0; j < B; ++j)
+		rw[j] = scc[v ^ (int64_t(1) << (j < N ? j : N - 1))];
+	int32_t hw[B];
+#pragma unroll
+	for (int j = 0; j < B; ++j)
+		hw[j] = h[rw[j]]; // every lane gathers, so the loads are issued together
+	bool blk = false;
+#pragma unroll
+	for (int j = 0; j < B; ++j)
+		blk |= j < N && ((mv >> j) & 1u) && rw[j] != r && hw[j] == 0;
+	if (blk)
+		blocked[r] = cur;
 }
 
 __global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32_t* __restrict__ scc,
 														  int32_t* __restrict__ h,
 														  const int32_t* __restrict__ blocked, int32_t cur,
-														  int32_t* __restrict__ labelled)
+														  int32_t* __restrict__ labelled, int32_t* __restrict__ total,
+														  int32_t S)
 {
+	if (total[0] >= S)
+		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	int got = 0;
 	if (v < n && scc[v] == (int32_t)v && h[v] == 0 && blocked[v] != cur)
@@ -241,6 +312,12 @@ __global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32
 	const int w = __popcll(__ballot(got));
 	if (lane_id() == 0 && w)
 		atomicAdd(labelled, w);
+}
+
+// the labelled count of a level into the running total (one thread, after the level)
+__global__ void k_height_total(const int32_t* __restrict__ labelled, int32_t* __restrict__ total)
+{
+	total[0] += labelled[0];
 }
 
 // ---- canonical order
@@ -328,29 +405,57 @@ struct pass_flags
 	}
 };
 
-// Runs a fixpoint pass in groups of `group` launches with one host round trip per
+// Runs a fixpoint pass in groups of launches with one host round trip per
 // group: pass p of a group reports a change in its own flag slot; the first pass
-// that changed nothing ends the loop (the passes after it in its group found the
-// fixpoint unchanged).  pass(flag) launches one pass.
+// that changed nothing ends the loop (the passes after it in its group return
+// at once, seeing their predecessor's flag clear).  pass(flag, prev) launches
+// one pass (prev: the previous pass's flag, null for a group's first).  The
+// first group has `hint` passes (what the same fixpoint needed last time, plus
+// one), later groups `group`; hint is updated to the passes used plus one.
 template <typename F>
-void fixpoint(ctx_t& c, int group, int max_passes, const char* what, F&& pass)
+void fixpoint(ctx_t& c, int& hint, int group, int max_passes, const char* what, F&& pass)
 {
-	constexpr int base = 8; // flag slots base .. base + group - 1
+	constexpr int base = 8, slots = 16; // flag slots base .. base + slots - 1
 	int32_t* d = c.flags.get() + base;
-	for (int done = 0;; done += group)
+	for (int done = 0;;)
 	{
-		ESL_HIP(hipMemsetAsync(d, 0, group * sizeof(int32_t), c.stream));
-		for (int p = 0; p < group; ++p)
-			pass(d + p);
-		ESL_HIP(hipMemcpyAsync(c.h_flags + base, d, group * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+		const int gsz = std::min(slots, std::max(2, done ? group : hint));
+		ESL_HIP(hipMemsetAsync(d, 0, gsz * sizeof(int32_t), c.stream));
+		for (int p = 0; p < gsz; ++p)
+			pass(d + p, p ? d + p - 1 : nullptr);
+		ESL_HIP(hipMemcpyAsync(c.h_flags + base, d, gsz * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
 		ESL_HIP(hipStreamSynchronize(c.stream));
-		for (int p = 0; p < group; ++p)
+		for (int p = 0; p < gsz; ++p)
 			if (!c.h_flags[base + p])
+			{
+				hint = done + p + 2;
 				return;
+			}
+		done += gsz;
 		if (done > max_passes)
 			throw error(ESL_E_DEVICE, std::string(what) + " did not converge");
 	}
 }
+
+// the unrolled width of the high-bit loops for N state bits
+int hmax_for(int N) { return N <= 6 ? 0 : N <= 14 ? 8 : N <= 22 ? 16 : 26; }
+
+#define ESL_HMAX_DISPATCH(N, KERNEL, ...)                                                                              \
+	switch (hmax_for(N))                                                                                               \
+	{                                                                                                                  \
+	case 0:                                                                                                            \
+		KERNEL<0>__VA_ARGS__;                                                                                          \
+		break;                                                                                                         \
+	case 8:                                                                                                            \
+		KERNEL<8>__VA_ARGS__;                                                                                          \
+		break;                                                                                                         \
+	case 16:                                                                                                           \
+		KERNEL<16>__VA_ARGS__;                                                                                         \
+		break;                                                                                                         \
+	default:                                                                                                           \
+		KERNEL<26>__VA_ARGS__;                                                                                         \
+		break;                                                                                                         \
+	}
 
 } // namespace
 
@@ -371,24 +476,30 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	ESL_HIP(hipMemsetAsync(g.scc.get(), 0xff, n * sizeof(int32_t), c.stream));
 
 	const int max_passes = 1 << 20;
-	const int pass_group = 4; // fixpoint passes per host round trip
+	const int pass_group = 8;	   // fixpoint passes per host round trip, after the first
+	int hint[3] = { 8, 8, 8 }; // first-group sizes of the three fixpoints
 	int64_t prev_left = n + 1;
 	for (int round = 0;; ++round)
 	{
 		// algorithmic bytes of one pass over the flip masks: mask (4 B/state) and the
 		// state labels it reads and writes
-		fixpoint(c, pass_group, max_passes, "trim", [&](int32_t* f) {
-			timed_launch(c, "k_trim", 12.0 * n,
-						 [&] { k_trim<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), f); });
+		fixpoint(c, hint[0], pass_group, max_passes, "trim", [&](int32_t* f, const int32_t* pf) {
+			timed_launch(c, "k_trim", 12.0 * n, [&] {
+				ESL_HMAX_DISPATCH(N, k_trim, <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), f, pf));
+			});
 		});
 		k_color_init<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get());
-		fixpoint(c, pass_group, max_passes, "colouring", [&](int32_t* f) {
-			timed_launch(c, "k_color", 16.0 * n,
-						 [&] { k_color<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), f); });
+		fixpoint(c, hint[1], pass_group, max_passes, "colouring", [&](int32_t* f, const int32_t* pf) {
+			timed_launch(c, "k_color", 16.0 * n, [&] {
+				ESL_HMAX_DISPATCH(N, k_color,
+								  <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), f, pf));
+			});
 		});
-		fixpoint(c, pass_group, max_passes, "closure", [&](int32_t* f) {
+		fixpoint(c, hint[2], pass_group, max_passes, "closure", [&](int32_t* f, const int32_t* pf) {
 			timed_launch(c, "k_close", 14.0 * n, [&] {
-				k_close<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), inscc.get(), f);
+				ESL_HMAX_DISPATCH(N, k_close,
+								  <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(),
+																inscc.get(), f, pf));
 			});
 		});
 		fl.clear(1);
@@ -420,28 +531,49 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	blocked.ensure(n);
 	ESL_HIP(hipMemsetAsync(hroot.get(), 0, n * sizeof(int32_t), c.stream));
 	ESL_HIP(hipMemsetAsync(blocked.get(), 0, n * sizeof(int32_t), c.stream));
-	int32_t labelled_total = 0, T = 0, cur = 0;
-	while (labelled_total < S)
+	// levels in groups of `level_group` per host round trip: slot hbase + k counts
+	// the roots labelled at the group's k-th level, slot htotal all so far
+	constexpr int level_group = 16, hbase = 32, htotal = 31;
+	int32_t* dl = c.flags.get() + hbase;
+	int32_t* dtot = c.flags.get() + htotal;
+	ESL_HIP(hipMemsetAsync(dtot, 0, sizeof(int32_t), c.stream));
+	int32_t labelled_total = 0, T = 0, cur = 0, H = 0;
+	while (!H)
 	{
-		++cur;
-		timed_launch(c, "k_height_check", 13.0 * n, [&] {
-			k_height_check<<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), hroot.get(), blocked.get(),
-														cur);
-		});
-		fl.clear(2);
-		timed_launch(c, "k_height_assign", 9.0 * n, [&] {
-			k_height_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), blocked.get(), cur, fl.d + 2);
-		});
+		ESL_HIP(hipMemsetAsync(dl, 0, level_group * sizeof(int32_t), c.stream));
+		for (int k = 0; k < level_group; ++k)
+		{
+			const int32_t lev = cur + 1 + k;
+			timed_launch(c, "k_height_check", 13.0 * n, [&] {
+				ESL_HMAX_DISPATCH(N, k_height_check,
+								  <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), hroot.get(),
+																blocked.get(), lev, dtot, S));
+			});
+			timed_launch(c, "k_height_assign", 9.0 * n, [&] {
+				k_height_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), hroot.get(), blocked.get(), lev, dl + k,
+															 dtot, S);
+			});
+			k_height_total<<<1, 1, 0, c.stream>>>(dl + k, dtot);
+		}
 		ESL_HIP(hipGetLastError());
-		const int32_t got = fl.read(2);
-		if (got == 0)
-			throw error(ESL_E_DEVICE, "condensation graph is not acyclic");
-		if (cur == 1)
-			T = got;
-		labelled_total += got;
+		ESL_HIP(hipMemcpyAsync(c.h_flags + hbase, dl, level_group * sizeof(int32_t), hipMemcpyDeviceToHost,
+							   c.stream));
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		for (int k = 0; k < level_group && !H; ++k)
+		{
+			const int32_t got = c.h_flags[hbase + k];
+			if (got == 0)
+				throw error(ESL_E_DEVICE, "condensation graph is not acyclic");
+			if (cur + 1 + k == 1)
+				T = got;
+			labelled_total += got;
+			if (labelled_total >= S)
+				H = cur + 1 + k;
+		}
+		cur += level_group;
 	}
 	g.T = T;
-	g.H = cur;
+	g.H = H;
 
 	// sort roots by key
 	dbuf<uint64_t> rkeys, rkeys_out;
