@@ -2775,20 +2775,34 @@ __global__ __launch_bounds__(1024) void k_w_orth(int64_t nU, int B, int R, int K
 
 // out[b * ostride + i] = sum over blocks of part[blk][b][i]  (one wave per
 // (i, b)); outt (if not null) gets the same values transposed, outt[i * B + b]
-__global__ __launch_bounds__(wave) void k_w_reduce(const double* __restrict__ part, int nblk, int B, int K,
-												   double* __restrict__ out, int64_t ostride, double* __restrict__ outt)
+// out[b][i] = sum over the blocks k of part[k][b][i] (i < K), one workgroup per
+// set: thread t adds block rows t / K, t / K + G, ... (G = 256 / K groups) of
+// column i = t % K, so each group reads the K contiguous values of a block
+// row; the groups' sums are then added in group order (a fixed order)
+constexpr int wred_threads = 256;
+
+__global__ __launch_bounds__(wred_threads) void k_w_reduce(const double* __restrict__ part, int nblk, int B, int K,
+														   double* __restrict__ out, int64_t ostride,
+														   double* __restrict__ outt)
 {
-	const int i = blockIdx.x;
-	const int b = blockIdx.y;
+	__shared__ double s_sum[wred_threads];
+	const int b = blockIdx.x;
+	const int G = wred_threads / K; // K <= wred_threads
+	const int i = threadIdx.x % K, gi = threadIdx.x / K;
 	double s = 0.0;
-	for (int k = threadIdx.x; k < nblk; k += wave)
-		s += part[((int64_t)k * B + b) * K + i];
-	s = wave_sum(s);
-	if (threadIdx.x == 0)
+	if (gi < G)
+		for (int k = gi; k < nblk; k += G)
+			s += part[((int64_t)k * B + b) * K + i];
+	s_sum[threadIdx.x] = s;
+	__syncthreads();
+	if ((int)threadIdx.x < K)
 	{
-		out[(int64_t)b * ostride + i] = s;
+		double t = 0.0;
+		for (int q = 0; q < G; ++q)
+			t += s_sum[q * K + threadIdx.x];
+		out[(int64_t)b * ostride + threadIdx.x] = t;
 		if (outt)
-			outt[(int64_t)i * B + b] = s;
+			outt[(int64_t)threadIdx.x * B + b] = t;
 	}
 }
 
@@ -3460,7 +3474,6 @@ struct gmres_w_t
 	dbuf<int32_t> live, act, kb;
 	gm_state_t st{};
 	std::vector<double> h_res;
-	std::vector<int32_t> h_live;
 	int iters = 0, cycles = 0;
 	double final_res = 0.0;
 
@@ -3471,8 +3484,10 @@ struct gmres_w_t
 			throw error(ESL_E_ARG, "the device-wide solver takes at most 1024 rate sets per call");
 		build_wnbr(c, nU, NB);
 		G = NB.wview();
-		R = B >= 256 ? 1 : 256 / B;
-		nred = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nU, R), 4096));
+		// reduction kernels: B x R threads per workgroup (up to 1024), at most 512
+		// workgroups (two per CU), so the per-block partial sums stay small
+		R = std::max(1, 1024 / B);
+		nred = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nU, R), 512));
 		const size_t cnt = (size_t)nU * B;
 		V.ensure((size_t)(m + 1) * cnt);
 		W.ensure(cnt);
@@ -3547,11 +3562,43 @@ struct gmres_w_t
 		return mx;
 	}
 
-	bool any_live()
+	// Liveness after Arnoldi step j is copied to pinned memory with an event and
+	// read one step later, so the host never waits for the step it just
+	// enqueued: the device runs step j while the host checks step j - 1.
+	struct live_mirror_t
 	{
-		read_host(c, h_live, live.get(), (size_t)B);
-		for (int32_t v : h_live)
-			if (v)
+		int32_t* h = nullptr; // [2][B]
+		hipEvent_t ev[2] = { nullptr, nullptr };
+		~live_mirror_t()
+		{
+			if (h)
+				(void)hipHostFree(h);
+			for (hipEvent_t e : ev)
+				if (e)
+					(void)hipEventDestroy(e);
+		}
+	} lm;
+
+	void post_live(int j)
+	{
+		if (!lm.h)
+		{
+			ESL_HIP(hipHostMalloc((void**)&lm.h, 2 * (size_t)B * sizeof(int32_t), hipHostMallocDefault));
+			for (hipEvent_t& e : lm.ev)
+				ESL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		}
+		ESL_HIP(hipMemcpyAsync(lm.h + (j & 1) * B, live.get(), (size_t)B * sizeof(int32_t), hipMemcpyDeviceToHost,
+							   c.stream));
+		ESL_HIP(hipEventRecord(lm.ev[j & 1], c.stream));
+	}
+
+	// whether any set was still live after step j (posted by post_live(j))
+	bool live_after(int j)
+	{
+		ESL_HIP(hipEventSynchronize(lm.ev[j & 1]));
+		const int32_t* f = lm.h + (j & 1) * B;
+		for (int b = 0; b < B; ++b)
+			if (f[b])
 				return true;
 		return false;
 	}
@@ -3593,7 +3640,7 @@ struct gmres_w_t
 			k_w_residual<<<gq, sblock, 0, c.stream>>>(G, nU, B, rt.get(), winvd.get(), wanc.get(), wrhs.get(),
 													  wx.get(), r.get());
 			k_w_norms<<<nred, bs, 0, c.stream>>>(nU, B, R, r.get(), wrhs.get(), part.get());
-			k_w_reduce<<<dim3(2, B), wave, 0, c.stream>>>(part.get(), nred, B, 2, rr.get(), 2, nullptr);
+			k_w_reduce<<<B, wred_threads, 0, c.stream>>>(part.get(), nred, B, 2, rr.get(), 2, nullptr);
 			k_gm_start<<<sb, 64, 0, c.stream>>>(B, m, st, rr.get(), tol, vscale.get());
 			ESL_HIP(hipGetLastError());
 			final_res = max_res();
@@ -3630,7 +3677,7 @@ struct gmres_w_t
 					k_w_dots<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, part.get(),
 														   live.get());
 				});
-				k_w_reduce<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nred, B, j + 1, c1.get(), m + 1,
+				k_w_reduce<<<B, wred_threads, 0, c.stream>>>(part.get(), nred, B, j + 1, c1.get(), m + 1,
 																   ct.get());
 				timed_launch(c, "k_w_orth", dbytes + (double)cnt * sizeof(T), [&] {
 					k_w_orth<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, ct.get(),
@@ -3640,19 +3687,28 @@ struct gmres_w_t
 				{
 					k_w_dots<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, part.get(),
 														   live.get());
-					k_w_reduce<<<dim3(j + 1, B), wave, 0, c.stream>>>(part.get(), nred, B, j + 1, c2.get(), m + 1,
+					k_w_reduce<<<B, wred_threads, 0, c.stream>>>(part.get(), nred, B, j + 1, c2.get(), m + 1,
 																	   ct.get());
 					k_w_orth<T><<<nred, bs, 0, c.stream>>>(nU, B, R, j + 1, W.get(), V.get(), vstride, ct.get(),
 														   part.get(), live.get());
 				}
-				k_w_reduce<<<dim3(1, B), wave, 0, c.stream>>>(part.get(), nred, B, 1, hn.get(), 1, nullptr);
+				k_w_reduce<<<B, wred_threads, 0, c.stream>>>(part.get(), nred, B, 1, hn.get(), 1, nullptr);
 				k_gm_givens<<<sb, 64, 0, c.stream>>>(B, m, j, st, 0.5 * tol, sizeof(T) == 4 ? inner_rtol : 0.0);
 				k_w_scale<T><<<gq, sblock, 0, c.stream>>>(cnt, B, W.get(), hn.get(), V.get() + (int64_t)(j + 1) * vstride,
 														  live.get());
 				ESL_HIP(hipGetLastError());
 				++iters;
 				k = j + 1;
-				if (!any_live() || iters >= max_iters)
+				post_live(j);
+				if (j > 0 && !live_after(j - 1))
+				{
+					// every set had stopped after step j - 1: step j changed nothing that
+					// is used (dead sets keep their column count), so it is not counted
+					--iters;
+					k = j;
+					break;
+				}
+				if (iters >= max_iters)
 					break;
 			}
 			// x += P^{-1} (sum_i y_i V_i) in FP64
