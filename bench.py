@@ -721,13 +721,13 @@ def workload_seeds(a, ctx):
             dr = DeviceArray.from_host(ctx, rates)
             df = DeviceArray(ctx, (B, tp.n), np.float64)
             d1 = DeviceArray(ctx, (1, tp.n), np.float64)
-            sched = "device-wide, restart 32"
+            sched = "device-wide, restart 64"
             if nU <= 4096:
                 path, restart, _ = tp.plan.autotune(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1)
                 sched = f"{SCHEDULE_NAMES[path]}, restart {restart}"
                 run = lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1, restart=0)  # noqa: E731
             else:
-                run = lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, restart=32)  # noqa: E731
+                run = lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, restart=0)  # noqa: E731
             t_batch, rep = _timed(run, 1)
             t_plain, _ = _timed(lambda: tp.plan.solve(1, dr.ptr, d1.ptr, precision=0, tol=1e-13), 1)
             xb0 = np.empty(tp.n)  # batched set 0 only
@@ -768,8 +768,9 @@ def workload_n22(a, ctx):
     dr = DeviceArray.from_host(ctx, rates)
     d64 = DeviceArray(ctx, (B, tp.n), np.float64)
     d32 = DeviceArray(ctx, (B, tp.n), np.float64)
-    # 64 sets of 2e5 unknowns: the device-wide schedule (every vector operation over all sets at once)
-    tp.plan.set_schedule(2, 32)
+    # 64 sets of 2e5-2e6 unknowns: the device-wide schedule (every vector operation over all sets at
+    # once) at its default restart length, 64
+    tp.plan.set_schedule(2, 64)
     steps = max(1, a.steps // 3)
     t64, r64 = _timed(lambda: tp.plan.solve(B, dr.ptr, d64.ptr, precision=0, tol=1e-12, restart=0), steps)
     print(f"n22: fp64 {t64:.3f} s per batch", file=sys.stderr, flush=True)

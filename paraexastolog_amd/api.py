@@ -269,7 +269,7 @@ class sparse_cs_matrix:
         return len(self.indptr) - 1
 
 
-def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=32, full=0, preconditioner=0, inner_rtol=None,
+def solve_opts(precision=0, tol=1e-13, max_iters=20000, restart=0, full=0, preconditioner=0, inner_rtol=None,
                exact_fallback=None):
     o = SolveOpts()
     lib().esl_solve_opts_default(C.byref(o))

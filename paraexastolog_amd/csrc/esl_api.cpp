@@ -428,7 +428,7 @@ void esl_solve_opts_default(esl_solve_opts* o)
 	o->precision = 0;
 	o->tol = 1e-13;
 	o->max_iters = 20000;
-	o->restart = 32;
+	o->restart = 0;
 	o->full = 0;
 	o->preconditioner = 0;
 	o->graph = 0;

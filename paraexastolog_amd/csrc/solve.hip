@@ -1024,7 +1024,8 @@ struct nbr_t
 
 constexpr int pblock = 256;
 constexpr int pwaves = pblock / wave;
-constexpr int pm_max = 64;
+constexpr int pm_max = 64;  // restart length cap of the per-set solvers (Hessenberg in LDS)
+constexpr int dw_max = 128; // restart length cap of the device-wide solver (Hessenberg in global memory)
 
 struct pgm_out_t
 {
@@ -4127,6 +4128,26 @@ struct plan_t
 };
 
 
+// Restart length of the device-wide solver for cb sets of nU unknowns: the
+// caller's, or by default 64, lowered while the basis in FP64 (what a stalled
+// FP32 solve falls back to) would exceed half of the free device memory.
+// Measured on MI355X, N=22 seed 0, 64 sets of 1,769,472 unknowns, FP32 cycles:
+// restart 16 / 24 / 32 / 48 / 64: 12 / 23 / 27 / 35 / 37 sets/s (711 / 312 /
+// 241 / 150 / 128 iterations per set); at 96 and 128 the cycles already end at
+// their accuracy target before the restart (128 iterations, 37.6 sets/s).
+int dw_restart(int requested, int64_t nU, int cb)
+{
+	if (requested > 0)
+		return std::min(std::max(requested, 2), dw_max);
+	size_t free_b = 0, total_b = 0;
+	if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+		free_b = 0;
+	int m = 64;
+	while (m > 16 && (double)(m + 1) * (double)nU * cb * sizeof(double) > 0.5 * (double)free_b)
+		m /= 2;
+	return m;
+}
+
 // throw_on_miss = false leaves sets above the tolerance to the caller (kr.missed)
 krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* rhs_p, double* z_p,
 							 const esl_solve_opts& opts, const nbr_store_t* prebuilt = nullptr, bool device_wide = false,
@@ -4137,7 +4158,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 	const int B = S.B;
 	const double tol = opts.tol > 0 ? opts.tol : 1e-13;
 	const int max_iters = opts.max_iters > 0 ? opts.max_iters : 20000;
-	const int restart = std::min(std::max(opts.restart > 0 ? opts.restart : 32, 2), 64);
+	const int restart = std::min(std::max(opts.restart > 0 ? opts.restart : 32, 2), dw_max);
 	if (!device_wide && nU <= per_set_max_unknowns(B))
 	{
 		const int m = std::min(restart, pm_max);
@@ -4186,7 +4207,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 			double cmax = 0.0;
 			if (opts.precision == 1)
 			{
-				gmres_w_t<float> gm(c, Sc, nb, nseg1, restart, ilu);
+				gmres_w_t<float> gm(c, Sc, nb, nseg1, dw_restart(opts.restart, nU, cb), ilu);
 				gm.enter(rhs_c, z_c);
 				done = gm.solve(tol, max_iters, opts.inner_rtol);
 				gm.leave(z_c);
@@ -4200,7 +4221,7 @@ krylov_result_t krylov_solve(ctx_t& c, const sys_t& S, int n_seg, const double* 
 			if (!done)
 			{
 				// FP64 from the current iterate (also what an FP32 solve that stalled falls back to)
-				gmres_w_t<double> gm(c, Sc, nb, nseg1, restart, ilu);
+				gmres_w_t<double> gm(c, Sc, nb, nseg1, dw_restart(opts.restart, nU, cb), ilu);
 				gm.enter(rhs_c, z_c);
 				const bool done64 = gm.solve(tol, std::max(max_iters - it, 1));
 				gm.leave(z_c);
@@ -5478,7 +5499,7 @@ void plan_set_schedule(void* p, int path, int restart)
 	if (path < 0)
 		P.tune = plan_t::tune_t{};
 	else
-		P.tune = plan_t::tune_t{ true, path, std::min(std::max(restart, 2), pm_max) };
+		P.tune = plan_t::tune_t{ true, path, std::min(std::max(restart, 2), path == 2 ? dw_max : pm_max) };
 }
 
 void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* method)
