@@ -721,14 +721,21 @@ def workload_seeds(a, ctx):
             dr = DeviceArray.from_host(ctx, rates)
             df = DeviceArray(ctx, (B, tp.n), np.float64)
             d1 = DeviceArray(ctx, (1, tp.n), np.float64)
-            sched = "device-wide, restart 64"
             if nU <= 4096:
                 path, restart, _ = tp.plan.autotune(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1)
                 sched = f"{SCHEDULE_NAMES[path]}, restart {restart}"
                 run = lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, graph=1, restart=0)  # noqa: E731
+                t_batch, rep = _timed(run, 1)
             else:
-                run = lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12, restart=0)  # noqa: E731
-            t_batch, rep = _timed(run, 1)
+                # device-wide: the faster of restart 32 and 64 (neither wins everywhere)
+                best = None
+                for m in (32, 64):
+                    t_m, rep_m = _timed(lambda: tp.plan.solve(B, dr.ptr, df.ptr, precision=1, tol=1e-12,  # noqa: B023
+                                                              restart=m), 1)
+                    if best is None or t_m < best[0]:
+                        best = (t_m, rep_m, m)
+                t_batch, rep, m = best
+                sched = f"device-wide, restart {m} (best of 32, 64)"
             t_plain, _ = _timed(lambda: tp.plan.solve(1, dr.ptr, d1.ptr, precision=0, tol=1e-13), 1)
             xb0 = np.empty(tp.n)  # batched set 0 only
             check(L.esl_memcpy_d2h(ctx.handle, xb0.ctypes.data_as(C.c_void_p), df.ptr, 8 * tp.n), ctx.handle)
@@ -769,7 +776,7 @@ def workload_n22(a, ctx):
     d64 = DeviceArray(ctx, (B, tp.n), np.float64)
     d32 = DeviceArray(ctx, (B, tp.n), np.float64)
     # 64 sets of 2e5-2e6 unknowns: the device-wide schedule (every vector operation over all sets at
-    # once) at its default restart length, 64
+    # once), restart 64 (measured faster than 32 on these systems: N=22 seed 8 377 vs 203 sets/s)
     tp.plan.set_schedule(2, 64)
     steps = max(1, a.steps // 3)
     t64, r64 = _timed(lambda: tp.plan.solve(B, dr.ptr, d64.ptr, precision=0, tol=1e-12, restart=0), steps)

@@ -144,10 +144,10 @@ typedef struct
 	int precision;		 /* 0 = FP64 Krylov solves; 1 = FP32 inner solves + FP64 residual refinement */
 	double tol;			 /* relative residual target of every linear solve (default 1e-13) */
 	int max_iters;		 /* Krylov iterations per solve before ESL_E_NUMERIC (default 20000) */
-	int restart;		 /* GMRES restart length; 0 (default): a plan's autotuned length, else 32 for
-						  * the per-set solvers (at most 64) and 64 for the device-wide solver (at
-						  * most 128; halved while its FP64 basis would exceed half the free device
-						  * memory) */
+	int restart;		 /* GMRES restart length: at most 64 for the per-set solvers, 128 for the
+						  * device-wide solver; 0 (default): a plan's autotuned length, else 32 (the
+						  * device-wide solver halves it while its FP64 basis would exceed half the
+						  * free device memory) */
 	int full;			 /* 1: also build solution_term / solution_nonterm (esl_solution_export) */
 	int preconditioner;	 /* 0 = default (ILU(0)), 1 = Jacobi, 2 = ILU(0) (level-scheduled) */
 	int graph;			 /* 1: capture the numeric sequence of a plan in a HIP graph and replay it */

@@ -2647,15 +2647,21 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_bwd(wnbr_t G, int B, int l, co
 __device__ __forceinline__ void w_block_sets(double v, int B, int R, double* s_red, double* __restrict__ part,
 											 int64_t slot)
 {
+	// the R values of set b (threads b, b + B, ...) folded in halves: a fixed
+	// order, log2(R) barriers instead of R serial additions by one thread
 	s_red[threadIdx.x] = v;
 	__syncthreads();
-	if ((int)threadIdx.x < B)
+	const int i = (int)threadIdx.x / B;
+	for (int r = R; r > 1;)
 	{
-		double acc = 0.0;
-		for (int k = 0; k < R; ++k)
-			acc += s_red[threadIdx.x + k * B];
-		part[slot] = acc;
+		const int h = (r + 1) / 2;
+		if (i < r - h)
+			s_red[threadIdx.x] += s_red[threadIdx.x + h * B];
+		__syncthreads();
+		r = h;
 	}
+	if ((int)threadIdx.x < B)
+		part[slot] = s_red[threadIdx.x];
 	__syncthreads();
 }
 
@@ -4129,21 +4135,24 @@ struct plan_t
 
 
 // Restart length of the device-wide solver for cb sets of nU unknowns: the
-// caller's, or by default 64, lowered while the basis in FP64 (what a stalled
-// FP32 solve falls back to) would exceed half of the free device memory.
-// Measured on MI355X, N=22 seed 0, 64 sets of 1,769,472 unknowns, FP32 cycles:
-// restart 16 / 24 / 32 / 48 / 64: 12 / 23 / 27 / 35 / 37 sets/s (711 / 312 /
-// 241 / 150 / 128 iterations per set); at 96 and 128 the cycles already end at
-// their accuracy target before the restart (128 iterations, 37.6 sets/s).
+// caller's (at most dw_max), or by default 32, lowered while the basis in FP64
+// (what a stalled FP32 solve falls back to) would exceed half of the free
+// device memory.  Longer restarts pay off on some systems and not on others
+// (MI355X, FP32 cycles + FP64 refinement, sets/s at restart 32 -> 64): N=22
+// seed 0, 1,769,472 unknowns: 27 -> 37 (241 -> 128 iterations per set; at 96
+// and 128 the cycles end at their FP32 accuracy target first, 37.6); N=22
+// seed 8, 221,184 unknowns: 203 -> 377; N=20 seeds 4 / 7, 524,288 unknowns:
+// 132 -> 79 and 236 -> 153; the uneven 8-network batch: 4.66 k -> 2.66 k.
+// So the default stays 32 and callers (or the autotuner) ask for more.
 int dw_restart(int requested, int64_t nU, int cb)
 {
+	int m = requested > 0 ? std::min(std::max(requested, 2), dw_max) : 32;
 	if (requested > 0)
-		return std::min(std::max(requested, 2), dw_max);
+		return m;
 	size_t free_b = 0, total_b = 0;
 	if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
 		free_b = 0;
-	int m = 64;
-	while (m > 16 && (double)(m + 1) * (double)nU * cb * sizeof(double) > 0.5 * (double)free_b)
+	while (m > 8 && (double)(m + 1) * (double)nU * cb * sizeof(double) > 0.5 * (double)free_b)
 		m /= 2;
 	return m;
 }
