@@ -572,32 +572,6 @@ __global__ __launch_bounds__(sblock) void k_zero2(int64_t count2, double2* __res
 // anchors: damped Jacobi sweeps p <- p/2 + (sum_t rate(t->s) p_t / D_s)/2 on
 // every terminal segment, then the argmax of each segment per set.
 
-__global__ __launch_bounds__(sblock) void k_anchor_sweep(sys_t S, const double* __restrict__ p_in,
-														 double* __restrict__ p_out)
-{
-	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
-	const int b = blockIdx.y;
-	if (u >= S.nUt)
-		return;
-	const double* r = S.rates + (int64_t)b * S.rate_stride;
-	const int32_t s = S.ulist[u];
-	const int32_t cls = S.ucls[u];
-	const double* pb = p_in + (int64_t)b * S.nUt;
-	double inflow = 0.0;
-	for (int i = 0; i < S.N; ++i)
-	{
-		const uint32_t t = (uint32_t)s ^ (1u << i);
-		if (!((S.mask[t] >> i) & 1u))
-			continue;
-		const int32_t ut = S.pos[t];
-		if (ut < 0 || ut >= S.nUt || S.ucls[ut] != cls)
-			continue;
-		inflow += rate_of(r, i, ((s >> i) & 1) != 0) * pb[ut];
-	}
-	const double d = out_rate_s((uint32_t)s, S.mask[s], r);
-	p_out[(int64_t)b * S.nUt + u] = 0.5 * pb[u] + 0.5 * inflow / d;
-}
-
 // one block per (segment, set): anchor = smallest unknown index of maximal p
 __global__ __launch_bounds__(sblock) void k_anchor_pick(int64_t nUt, const int32_t* __restrict__ seg_off,
 														const double* __restrict__ p, int n_seg,
@@ -1009,6 +983,43 @@ struct nbr_t
 	const uint32_t* rec; // [nnz] neighbour position | rate code << 16
 };
 
+// One damped Jacobi sweep of the anchor choice (the sweeps and the pick are
+// described with k_anchor_pick).  The in-neighbours are read from the coupling lists (a terminal unknown's
+// list holds exactly its in-neighbours in its own SCC, with their rate codes),
+// four at a time, so their loads are in flight together.
+__global__ __launch_bounds__(sblock) void k_anchor_sweep(sys_t S, nbr_t G, const double* __restrict__ p_in,
+														 double* __restrict__ p_out)
+{
+	const int64_t u = (int64_t)blockIdx.x * sblock + threadIdx.x;
+	const int b = blockIdx.y;
+	if (u >= S.nUt)
+		return;
+	const double* r = S.rates + (int64_t)b * S.rate_stride;
+	const int32_t s = S.ulist[u];
+	const double* pb = p_in + (int64_t)b * S.nUt;
+	double inflow = 0.0;
+	int32_t e = G.ptr[u];
+	const int32_t e1 = G.ptr[u + 1];
+	for (; e + 4 <= e1; e += 4)
+	{
+		double rv[4], pv[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+		{
+			rv[k] = r[G.nr[e + k] & 63];
+			pv[k] = pb[G.nu[e + k]];
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			inflow += rv[k] * pv[k];
+	}
+	for (; e < e1; ++e)
+		inflow += r[G.nr[e] & 63] * pb[G.nu[e]];
+	const double d = out_rate_s((uint32_t)s, S.mask[s], r);
+	p_out[(int64_t)b * S.nUt + u] = 0.5 * pb[u] + 0.5 * inflow / d;
+}
+
+
 // ============================================================================
 // One workgroup per rate set: the whole restarted GMRES of that set in one
 // launch.  For systems up to a few tens of thousands of unknowns per set this
@@ -1040,7 +1051,8 @@ struct pgm_scratch_t
 {
 	double* r;	   // [B][nU] residual / FP64 correction
 	double* udiag; // [B][nU] ILU(0) pivots
-	double* t;	   // [B][nU] preconditioned basis vector (FP64: the preconditioner is exact to FP64)
+	double* t;	   // [B][nU] preconditioned basis vector, FP64 here (the LDS-resident kernels keep it, and
+				   // the inverse pivots, in the basis precision)
 	void* V;	   // [B][m+1][nU] Krylov basis (FP64 or FP32)
 };
 
@@ -4568,7 +4580,9 @@ void plan_enqueue_setup(ctx_t& c, plan_t& P, int B, const double* d_rates, const
 			double *pa = P.rb.p0, *pb = P.rb.p1;
 			for (int it = 0; it < P.anchor_sweeps; ++it)
 			{
-				k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, pa, pb);
+				timed_launch(c, "k_anchor_sweep", 16.0 * B * nUt, [&] {
+					k_anchor_sweep<<<gt, sblock, 0, c.stream>>>(S, P.nbr.view(), pa, pb);
+				});
 				std::swap(pa, pb);
 			}
 			if (pa != P.rb.p0)
@@ -5630,13 +5644,16 @@ void solution_full(ctx_t& c, graph_t& g, const double* d_rates, const esl_solve_
 		S.rates = d_rates;
 		S.rate_stride = 2 * N;
 		S.transpose = 0;
+		nbr_store_t nb; // the coupling lists serve the anchor sweeps and the solve
+		build_nbr(c, S, nb);
 		dbuf<double> p0, p1;
 		p0.ensure(nUt);
 		p1.ensure(nUt);
 		write_dev(c, p0.get(), std::vector<double>(nUt, 1.0));
 		for (int it = 0; it < 32; ++it)
 		{
-			k_anchor_sweep<<<dim3(ceil_div(nUt, sblock), 1), sblock, 0, c.stream>>>(S, p0.get(), p1.get());
+			k_anchor_sweep<<<dim3(ceil_div(nUt, sblock), 1), sblock, 0, c.stream>>>(S, nb.view(), p0.get(),
+																				   p1.get());
 			std::swap(p0, p1);
 		}
 		k_anchor_pick<<<dim3(n_seg, 1), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), p0.get(), n_seg, anchor.get());
@@ -5653,7 +5670,7 @@ void solution_full(ctx_t& c, graph_t& g, const double* d_rates, const esl_solve_
 		k_rhs<<<gu, sblock, 0, c.stream>>>(S, n_seg, nullptr, rhs.get());
 		k_zero<double><<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, z.get());
 		ESL_HIP(hipGetLastError());
-		krylov_solve(c, S, n_seg, rhs.get(), z.get(), opts);
+		krylov_solve(c, S, n_seg, rhs.get(), z.get(), opts, &nb);
 		k_seg_sum<<<dim3(n_seg, 1), sblock, 0, c.stream>>>(nUt, d_seg_off.get(), z.get(), invd.get(), n_seg,
 														  segsum.get());
 		k_times_invd<<<ceil_div(nUt, sblock), sblock, 0, c.stream>>>(nUt, z.get(), invd.get(), pi.get());
