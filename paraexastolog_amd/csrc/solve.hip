@@ -2577,9 +2577,11 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_pivot(wnbr_t G, int B, int l, 
 template <typename X>
 __global__ __launch_bounds__(sblock) void k_w_ilu_fwd(wnbr_t G, int B, int l, const double* __restrict__ rt,
 													  const uint8_t* __restrict__ wanc,
-													  const double* __restrict__ ud, X* __restrict__ v,
+													  const double* __restrict__ ud, const X* src, X* v,
 													  const int32_t* __restrict__ live)
 {
+	// v's rows of level l from src's (src may be v): the forward sweep reads every
+	// row once, so it also moves the vector into the sweep's output
 	const int64_t q0 = G.L.off[l];
 	const int64_t i = (int64_t)blockIdx.x * sblock + threadIdx.x;
 	if (i >= (G.L.off[l + 1] - q0) * B)
@@ -2588,9 +2590,14 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_fwd(wnbr_t G, int B, int l, co
 	int b;
 	w_row_set(i, B, q0, q, b);
 	const int64_t at = q * B + b;
-	if (!live[b] || wanc[at])
+	if (!live[b])
 		return;
-	double y = (double)v[at];
+	if (wanc[at])
+	{
+		v[at] = src[at];
+		return;
+	}
+	double y = (double)src[at];
 	int32_t e = G.ptr[q];
 	const int32_t e1 = G.mid[q];
 	for (; e + 4 <= e1; e += 4) // four neighbours' loads in flight; the sum keeps its order
@@ -3622,11 +3629,14 @@ struct gmres_w_t
 		return false;
 	}
 
+	// v <- P^{-1} src for the sets with flags[b] != 0 (src may be v)
 	template <typename X>
-	void precondition(X* v, const int32_t* flags) // in place: v <- P^{-1} v for the sets with flags[b] != 0
+	void precondition(const X* src, X* v, const int32_t* flags)
 	{
 		if (!ilu)
 		{
+			if (src != v)
+				k_w_copy_live<X, X><<<grid(nU * B), sblock, 0, c.stream>>>(nU * B, B, src, v, flags);
 			k_w_jacobi<X><<<grid(nU * B), sblock, 0, c.stream>>>(nU * B, B, winvd.get(), v, flags);
 			return;
 		}
@@ -3635,7 +3645,7 @@ struct gmres_w_t
 			const int cntl = G.L.off[l + 1] - G.L.off[l];
 			if (cntl)
 				k_w_ilu_fwd<X><<<grid((int64_t)cntl * B), sblock, 0, c.stream>>>(G, B, l, rt.get(), wanc.get(),
-																				 ud.get(), v, flags);
+																				 ud.get(), src, v, flags);
 		}
 		for (int l = G.L.n - 1; l >= 0; --l)
 		{
@@ -3683,9 +3693,8 @@ struct gmres_w_t
 			int k = 0;
 			for (int j = 0; j < m; ++j)
 			{
-				T* Vj = V.get() + (int64_t)j * vstride;
-				k_w_copy_live<T, T><<<gq, sblock, 0, c.stream>>>(cnt, B, Vj, t.get(), live.get());
-				precondition<T>(t.get(), live.get());
+				const T* Vj = V.get() + (int64_t)j * vstride;
+				precondition<T>(Vj, t.get(), live.get());
 				// algorithmic bytes: t read, W written, coupling structure and 1/D' read
 				timed_launch(c, "k_w_apply", (double)cnt * (2.0 * sizeof(T) + 8.0) + 12.0 * nU + 5.0 * NB.nnz, [&] {
 					k_w_apply<T, T><<<gq, sblock, 0, c.stream>>>(G, nU, B, rt.get(), winvd.get(), wanc.get(), t.get(),
@@ -3735,7 +3744,7 @@ struct gmres_w_t
 			k_w_transpose_y<<<grid((int64_t)B * (m + 1)), sblock, 0, c.stream>>>(B, m, c1.get(), yt.get());
 			k_zero<double><<<gq, sblock, 0, c.stream>>>(cnt, r.get());
 			k_w_combine<T><<<gq, sblock, 0, c.stream>>>(cnt, B, k, V.get(), vstride, yt.get(), r.get(), kb.get());
-			precondition<double>(r.get(), act.get());
+			precondition<double>(r.get(), r.get(), act.get());
 			if (sizeof(T) == 4) // keep the correction in t until the next residual has been checked
 				k_axpy1_keep<<<gq, sblock, 0, c.stream>>>(cnt, r.get(), wx.get(), keep.get());
 			else
