@@ -49,6 +49,7 @@
 #include <cstdio>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 #include "common.hip.hpp"
 #include "esl_internal.hpp"
@@ -5254,6 +5255,49 @@ void ensure_aux(ctx_t& c, int n)
 	}
 }
 
+void many_per_set(ctx_t& c, const std::vector<int>& conc, void* const* plans, const int* B,
+				  const double* const* d_rates, double* const* d_final, const esl_solve_opts& o,
+				  std::vector<esl_solve_opts>& po, esl_solve_report* reps);
+
+// A context of its own (stream, events, flags; profiling off) for a host thread
+// that drives one device-wide solve while the calling thread drives the rest.
+struct child_ctx_t
+{
+	ctx_t c;
+	explicit child_ctx_t(const ctx_t& parent)
+	{
+		c.device = parent.device;
+		c.lds_per_block = parent.lds_per_block;
+		c.wall_khz = parent.wall_khz;
+		ESL_HIP(hipSetDevice(c.device));
+		ESL_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+		for (auto& e : c.ev)
+			ESL_HIP(hipEventCreate(&e));
+		c.flags.ensure(64);
+		ESL_HIP(hipHostMalloc((void**)&c.h_flags, 64 * sizeof(int32_t), hipHostMallocDefault));
+	}
+	~child_ctx_t()
+	{
+		(void)hipSetDevice(c.device);
+		if (c.stream)
+		{
+			(void)hipStreamSynchronize(c.stream);
+			(void)hipStreamDestroy(c.stream);
+		}
+		for (auto& e : c.ev)
+			if (e)
+				(void)hipEventDestroy(e);
+		if (c.h_flags)
+			(void)hipHostFree(c.h_flags);
+	}
+	child_ctx_t(const child_ctx_t&) = delete;
+	child_ctx_t& operator=(const child_ctx_t&) = delete;
+};
+
+// Several plans (distinct networks) in one call.  Plans solved per set run
+// concurrently on forked streams of the calling thread (one HIP graph with
+// graph = 1); each plan that needs the host-driven device-wide solver runs at
+// the same time on a host thread and stream of its own.
 void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const double* const* d_rates,
 					 double* const* d_final, const esl_solve_opts& o_in, esl_solve_report* reps)
 {
@@ -5262,7 +5306,7 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		o.tol = 1e-13;
 	if (o.max_iters <= 0)
 		o.max_iters = 20000;
-	std::vector<int> conc;
+	std::vector<int> conc, wide;
 	std::vector<esl_solve_opts> po(np, o); // per plan: the tuned restart length when asked for
 	for (int i = 0; i < np; ++i)
 	{
@@ -5276,8 +5320,53 @@ void plan_solve_many(ctx_t& c, int np, void* const* plans, const int* B, const d
 		if (P.nU == 0 || (tuned ? P.tune.path != 2 : P.nU <= per_set_max_unknowns(B[i])))
 			conc.push_back(i);
 		else // one large system: its device-wide solver is driven from the host
-			plan_run(c, P, B[i], d_rates[i], d_final[i], o_in, reps[i]);
+			wide.push_back(i);
 	}
+	// the device-wide plans, each on a host thread of its own, joined before return
+	std::vector<std::unique_ptr<child_ctx_t>> kids;
+	for (size_t k = 0; k < wide.size(); ++k)
+		kids.push_back(std::make_unique<child_ctx_t>(c));
+	std::vector<std::exception_ptr> errs(wide.size());
+	std::vector<std::thread> workers;
+	for (size_t k = 0; k < wide.size(); ++k)
+		workers.emplace_back([&, k] {
+			try
+			{
+				const int i = wide[k];
+				ESL_HIP(hipSetDevice(kids[k]->c.device));
+				plan_run(kids[k]->c, *static_cast<plan_t*>(plans[i]), B[i], d_rates[i], d_final[i], o_in, reps[i]);
+			}
+			catch (...)
+			{
+				errs[k] = std::current_exception();
+			}
+		});
+	auto join_all = [&] {
+		for (std::thread& t : workers)
+			if (t.joinable())
+				t.join();
+		workers.clear();
+	};
+	try
+	{
+		many_per_set(c, conc, plans, B, d_rates, d_final, o, po, reps);
+	}
+	catch (...)
+	{
+		join_all();
+		throw;
+	}
+	join_all();
+	for (const std::exception_ptr& e : errs)
+		if (e)
+			std::rethrow_exception(e);
+}
+
+// the per-set plans of plan_solve_many: concurrent on forked streams
+void many_per_set(ctx_t& c, const std::vector<int>& conc, void* const* plans, const int* B,
+				  const double* const* d_rates, double* const* d_final, const esl_solve_opts& o,
+				  std::vector<esl_solve_opts>& po, esl_solve_report* reps)
+{
 	if (conc.empty())
 		return;
 	for (int i : conc)

@@ -150,14 +150,18 @@ def test_synthetic_vs_oracle(ctx, cfg, seed):
     st = initial_state(pm.nodes, fixed, vals)
     s = solver(ctx, t, g, rates[0], st)
     s.solve()
+    x = s.final_state
+    assert abs(x.sum() - 1) < 1e-12 and x.min() >= -1e-14
+    assert s.report["residual"] <= 1e-13
     if cfg == "c0":
         ref = P.solve(n, ip, rows, rates[0], x0, graph, full=True).final_state
-        assert l1(s.final_state, ref) <= 1e-12
-    else:
-        # property checks at N=16 (oracle direct solve is too slow for every seed)
-        x = s.final_state
-        assert abs(x.sum() - 1) < 1e-12 and x.min() >= -1e-14
-        assert s.report["residual"] <= 1e-13
+        assert l1(x, ref) <= 1e-12
+    elif seed != 2:
+        # the oracle's reachability-pruned solve (seconds at N=16); c2 seed 2 has a
+        # transient SCC of 62,496 states whose direct sparse solve takes minutes,
+        # so that seed keeps the property checks above
+        ref = P.Pipeline(om, rates[0], x0, full=False).final_state
+        assert l1(x, ref) <= 1e-12
 
 
 @pytest.mark.parametrize("name", ["solver.toy", "solver.toy2", "solver.toy3"])
