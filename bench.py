@@ -59,7 +59,7 @@ def parse():
                          "3 per-set LDS with the basis in global memory, 4/5 per-set at 3/2 workgroups per CU)")
     ap.add_argument("--workload", default="n20", choices=["n20", "n16", "uneven", "n22", "warm", "seeds"],
                     help="n20 (default) is the headline metric; the others print their own JSON line")
-    ap.add_argument("--epochs", type=int, default=30, help="training epochs of the warm workload")
+    ap.add_argument("--epochs", type=int, default=100, help="training epochs of the warm workload")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the run, copy 1 GiB device to device three times (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
