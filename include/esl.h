@@ -97,7 +97,9 @@ esl_status esl_initial_state(const esl_model* m, const char* const* fixed_names,
  * the device, from the rule truth tables, the per-state flip mask and the
  * CSC table over source states.  A zero rate removes that direction.  The
  * rates only matter through their zero pattern; they are kept for esl_solve
- * when it is called with rates == NULL. */
+ * when it is called with rates == NULL.  Networks of 1..27 nodes (the table's
+ * N 2^(N-1) entries are addressed with 32-bit indices); larger ones are
+ * ESL_E_ARG. */
 esl_status esl_table_build(esl_ctx* ctx, const esl_model* m, const double* rates /* 2N */, esl_graph** out);
 void esl_graph_destroy(esl_graph* g);
 
