@@ -59,7 +59,9 @@ const char* esl_version(void);
  * Replaces model_builder::construct_model (cuda/src/model.h:41,
  * model.cpp:80-125) with the formula grammar of BoolExprParser
  * (cuda/src/boolstuff/BoolExprParser.cpp:64-200).  Each rule is kept as a
- * truth table over the nodes it reads. */
+ * truth table over the nodes it reads.  Models of more than 26 nodes (2^N
+ * states as 32-bit integers, the table in int32) are ESL_E_ARG, rules reading
+ * more than 24 nodes ESL_E_PARSE. */
 esl_status esl_model_load(const char* bnet_path, esl_model** out);
 esl_status esl_model_parse(const char* bnet_text, esl_model** out);
 void esl_model_destroy(esl_model* m);
@@ -97,9 +99,7 @@ esl_status esl_initial_state(const esl_model* m, const char* const* fixed_names,
  * the device, from the rule truth tables, the per-state flip mask and the
  * CSC table over source states.  A zero rate removes that direction.  The
  * rates only matter through their zero pattern; they are kept for esl_solve
- * when it is called with rates == NULL.  Networks of 1..27 nodes (the table's
- * N 2^(N-1) entries are addressed with 32-bit indices); larger ones are
- * ESL_E_ARG. */
+ * when it is called with rates == NULL. */
 esl_status esl_table_build(esl_ctx* ctx, const esl_model* m, const double* rates /* 2N */, esl_graph** out);
 void esl_graph_destroy(esl_graph* g);
 

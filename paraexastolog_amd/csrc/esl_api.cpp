@@ -69,9 +69,6 @@ esl_status guarded(esl_ctx* ctx, F&& f)
 	}
 }
 
-// largest network: N 2^(N-1) transitions must fit a 32-bit index (N = 27: 1.8e9)
-constexpr int max_nodes = 27;
-
 void require(bool cond, int code, const std::string& what)
 {
 	if (!cond)
@@ -320,13 +317,8 @@ esl_status esl_table_build(esl_ctx* ctx, const esl_model* m, const double* rates
 	return guarded(ctx, [&] {
 		require(ctx && m && rates && out, ESL_E_ARG, "null argument");
 		set_device(ctx);
-		const int N = (int)m->m.nodes.size();
-		// the transition table has N 2^(N-1) entries at most, addressed with 32-bit
-		// indices (the reference's index_t is int as well)
-		require(N >= 1 && N <= max_nodes, ESL_E_ARG,
-				"networks of 1.." + std::to_string(max_nodes) + " nodes are supported, this one has " +
-					std::to_string(N));
 		auto g = std::make_unique<esl_graph>();
+		const int N = (int)m->m.nodes.size();
 		g->g.rates.assign(rates, rates + 2 * N);
 		g->g.up_ok = g->g.down_ok = 0;
 		for (int i = 0; i < N; ++i)

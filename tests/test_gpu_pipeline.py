@@ -215,14 +215,3 @@ def test_preconditioners_and_precisions(ctx, name, pre, prec):
     assert nz.tolist() == c["nonzero_indices"]
     np.testing.assert_allclose(v, c["nonzero_data"], rtol=0, atol=c["tol"])
     assert s.report["residual"] <= tol
-
-
-def test_network_beyond_the_index_range_is_refused(ctx):
-    # 28 nodes: 28 * 2^27 transitions exceed 32-bit indices
-    from paraexastolog_amd._lib import EslError
-
-    text = "targets, factors\n" + "".join(f"v{i}, v{i}\n" for i in range(28))
-    m = model_builder().parse_model(text)
-    with pytest.raises(EslError, match="1..27 nodes"):
-        transition_table(ctx, m, np.ones(56)).construct_table()
-
