@@ -205,6 +205,40 @@ def test_solve_many_matches_single_plans(ctx, graph):
             assert reps[k]["residual"] <= 1e-12
 
 
+def test_solve_many_error_on_a_device_wide_plan(ctx):
+    # c2 seed 37 (33,280 unknowns) runs device-wide on a host thread of its own;
+    # one Krylov iteration without the exact fallback misses the tolerance there,
+    # the error reaches the caller after every thread has joined, and the context
+    # then solves the same plans normally
+    from paraexastolog_amd._lib import EslError
+
+    specs = [("c0", 1, 5), ("c2", 37, 2)]
+    plans, nsets, drs, dfs, singles, keep = [], [], [], [], [], []
+    for cfg, seed, B in specs:
+        net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
+        n = 1 << net.N
+        dx0 = DeviceArray.from_host(ctx, x0)
+        dr = DeviceArray.from_host(ctx, rates)
+        df = DeviceArray(ctx, (B, n), np.float64)
+        ds = DeviceArray(ctx, (B, n), np.float64)
+        p = solve_plan(ctx, t, dx0.ptr)
+        p.solve(B, dr.ptr, ds.ptr, precision=1, tol=1e-12)
+        plans.append(p)
+        nsets.append(B)
+        drs.append(dr.ptr)
+        dfs.append(df.ptr)
+        singles.append(ds)
+        keep += [t, g, dx0, dr, df]
+    assert plans[1].info()["unknowns"] > 4096
+    with pytest.raises(EslError) as e:
+        solve_plan.solve_many(ctx, plans, nsets, drs, dfs, precision=1, tol=1e-12, max_iters=1, exact_fallback=0)
+    assert "ESL_E_NUMERIC" in str(e.value)
+    reps = solve_plan.solve_many(ctx, plans, nsets, drs, dfs, precision=1, tol=1e-12)
+    for k in range(len(specs)):
+        assert np.abs(keep[5 * k + 4].download() - singles[k].download()).sum(axis=1).max() <= 1e-10
+        assert reps[k]["residual"] <= 1e-12
+
+
 @pytest.mark.parametrize("cfg,seed,B", [("c0", 1, 8), ("c2", 11, 4)])
 def test_autotune_choice_keeps_results(ctx, cfg, seed, B):
     net, rates, m, t, g, x0 = setup_net(ctx, cfg, seed, B)
