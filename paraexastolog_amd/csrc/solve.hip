@@ -2550,7 +2550,7 @@ __global__ __launch_bounds__(sblock) void k_w_jacobi(int64_t count, int B, const
 	v[i] = (X)((double)v[i] * winvd[i]);
 }
 
-// ILU(0) pivots of the rows of one level (the previous level's are final)
+// inverse ILU(0) pivots of the rows of one level (the previous level's are final)
 __global__ __launch_bounds__(sblock) void k_w_ilu_pivot(wnbr_t G, int B, int l, const double* __restrict__ rt,
 														const double* __restrict__ winvd,
 														const uint8_t* __restrict__ wanc, double* __restrict__ ud)
@@ -2570,9 +2570,9 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_pivot(wnbr_t G, int B, int l, 
 			const int code = G.nr[e];
 			const int64_t k = (int64_t)G.nu[e] * B + b;
 			if ((code & nbr_rev_flag) && !wanc[k])
-				d -= rt[(code & 63) * B + b] * rt[((code & 63) ^ 1) * B + b] / ud[k];
+				d -= rt[(code & 63) * B + b] * rt[((code & 63) ^ 1) * B + b] * ud[k];
 		}
-	ud[at] = d;
+	ud[at] = 1.0 / d; // the sweeps multiply by inverse pivots (no division per neighbour)
 }
 
 template <typename X>
@@ -2614,12 +2614,12 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_fwd(wnbr_t G, int B, int l, co
 		}
 #pragma unroll
 		for (int k = 0; k < 4; ++k)
-			y += rv[k] / uv[k] * vv[k];
+			y += rv[k] * uv[k] * vv[k];
 	}
 	for (; e < e1; ++e)
 	{
 		const int64_t k = (int64_t)G.nu[e] * B + b;
-		y += rt[(G.nr[e] & 63) * B + b] / ud[k] * (double)v[k];
+		y += rt[(G.nr[e] & 63) * B + b] * ud[k] * (double)v[k];
 	}
 	v[at] = (X)y;
 }
@@ -2658,7 +2658,7 @@ __global__ __launch_bounds__(sblock) void k_w_ilu_bwd(wnbr_t G, int B, int l, co
 	}
 	for (; e < e1; ++e)
 		z += rt[(G.nr[e] & 63) * B + b] * (double)v[(int64_t)G.nu[e] * B + b];
-	v[at] = (X)(z / ud[at]);
+	v[at] = (X)(z * ud[at]);
 }
 
 // per-set sums over rows: a block of R * B threads, thread t works for set
@@ -3494,7 +3494,7 @@ struct gmres_w_t
 	dbuf<T> V, W;
 	dbuf<T> t;		  // preconditioned Arnoldi vector
 	dbuf<double> keep; // FP32 cycles: the last correction, undone if it raised the residual
-	dbuf<double> r, ud, wrhs, wx, winvd;
+	dbuf<double> r, ud, wrhs, wx, winvd; // ud: inverse ILU(0) pivots
 	dbuf<double> rt, ct, yt; // rates, Gram-Schmidt coefficients, combination weights: [index][set]
 	dbuf<uint8_t> wanc;
 	dbuf<double> H, cs, sn, g, c1, c2, hn, beta, bnorm, res, vscale, part, rr, res0;
