@@ -84,11 +84,13 @@ TABLE_KERNELS = ("k_flip_mask", "k_table_fill")
 MASK_PASS_KERNELS = ("k_trim", "k_color", "k_close", "k_height_check")
 
 
-def stage_bandwidth(stats, n, m):
+def stage_bandwidth(stats, n, m, counts=None):
     """Per-stage device time, algorithmic bytes and GB/s of the table and graph
-    stages.  Graph stage: passes over the per-state flip masks (4 B per state
-    each, plus the labels they read and write); also the compulsory-bytes form
-    4 n * passes + 3 * 4 n (mask per pass; labels, heights, order written once)."""
+    stages.  Graph stage: passes over the per-state flip masks, counted by the
+    library (`counts`, esl_graph_stage_counts; passes that found their
+    predecessor converged return at once and are not counted), with the
+    compulsory-bytes form 4 n * passes + 3 * 4 n (mask per pass; labels,
+    heights, order written once) as the algorithmic bytes."""
     tab = [s for s in stats if s[0] in TABLE_KERNELS]
     gra = [s for s in stats if s[0] not in TABLE_KERNELS and s[0].startswith("k_")]
     out = {}
@@ -98,9 +100,15 @@ def stage_bandwidth(stats, n, m):
         out[key] = {"kernel_ms": ms, "algorithmic_GB": by / 1e9, "GBps": by / (ms * 1e-3) / 1e9 if ms > 0 else None,
                     "kernels": {k[0]: {"launches": k[1], "ms": k[2]} for k in ks}}
     out["table"]["compulsory_GB"] = (4.0 * (n + 1) + 8.0 * m + 4.0 * n) / 1e9  # indptr, rows, cols, masks written
-    passes = sum(k[1] for k in gra if k[0] in MASK_PASS_KERNELS)
-    out["graph"]["mask_passes"] = passes
-    out["graph"]["compulsory_GB"] = (4.0 * n * passes + 12.0 * n) / 1e9
+    passes = counts["mask_passes"] if counts else sum(k[1] for k in gra if k[0] in MASK_PASS_KERNELS)
+    g = out["graph"]
+    g["mask_passes"] = passes
+    g["compulsory_GB"] = (4.0 * n * passes + 12.0 * n) / 1e9
+    g["algorithmic_GB"] = g["compulsory_GB"]
+    g["GBps"] = g["compulsory_GB"] / (g["kernel_ms"] * 1e-3) if g["kernel_ms"] > 0 else None
+    if counts:
+        g["scc_rounds"] = counts["scc_rounds"]
+        g["host_round_trips"] = counts["host_round_trips"]
     return out
 
 
@@ -315,7 +323,10 @@ def main():
     stage_ms = table.stage_ms()
     stage_stats = profile_stats(L, ctx)
     check(L.esl_profile(ctx.handle, 0), ctx.handle)
-    stages = stage_bandwidth(stage_stats, n, int(sizes[1]))
+    cnt = np.zeros(3, dtype=np.int64)
+    check(L.esl_graph_stage_counts(table.handle, cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+    stages = stage_bandwidth(stage_stats, n, int(sizes[1]),
+                             {"mask_passes": int(cnt[0]), "scc_rounds": int(cnt[1]), "host_round_trips": int(cnt[2])})
 
     def dalloc(nbytes):
         p = C.c_void_p()

@@ -127,6 +127,12 @@ esl_status esl_graph_import(esl_ctx* ctx, esl_graph* g, const int32_t* reordered
  * [5] = N (nodes), [6] = max SCC height. */
 esl_status esl_graph_query(const esl_graph* g, int64_t sizes[7]);
 
+/* Counters of the last esl_graph_reorganize: counts[0] = passes over the
+ * per-state flip masks (trimming, colouring and closure passes and height
+ * levels), [1] = SCC rounds, [2] = host round trips.  No reference
+ * counterpart (diagnostics of the graph stage). */
+esl_status esl_graph_stage_counts(const esl_graph* g, int64_t counts[3]);
+
 /* Table members indptr (n+1), rows (nnz, target states), cols (nnz, source
  * states); any pointer may be NULL. */
 esl_status esl_table_export(esl_ctx* ctx, const esl_graph* g, int32_t* indptr, int32_t* rows, int32_t* cols);

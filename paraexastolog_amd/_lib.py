@@ -91,6 +91,7 @@ SIGNATURES = [
     ("esl_graph_reorganize", C.c_int, [_VP, _VP]),
     ("esl_graph_import", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), C.c_int64, C.c_int32]),
     ("esl_graph_query", C.c_int, [_VP, _P(C.c_int64)]),
+    ("esl_graph_stage_counts", C.c_int, [_VP, _P(C.c_int64)]),
     ("esl_table_export", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
     ("esl_graph_export", C.c_int, [_VP, _VP, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
     ("esl_solve_opts_default", None, [_P(SolveOpts)]),

@@ -255,6 +255,12 @@ class transition_graph:
         self.terminals_count = int(tc.value)
         self.max_height = int(s[6])
 
+    def stage_counts(self):
+        """Counters of the graph stage (include/esl.h esl_graph_stage_counts)."""
+        a = np.zeros(3, dtype=np.int64)
+        check(lib().esl_graph_stage_counts(self.table.handle, a.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {"mask_passes": int(a[0]), "scc_rounds": int(a[1]), "host_round_trips": int(a[2])}
+
 
 @dataclass
 class sparse_cs_matrix:

@@ -355,6 +355,16 @@ esl_status esl_graph_import(esl_ctx* ctx, esl_graph* g, const int32_t* reordered
 	});
 }
 
+esl_status esl_graph_stage_counts(const esl_graph* g, int64_t counts[3])
+{
+	return guarded(nullptr, [&] {
+		require(g && counts, ESL_E_ARG, "null argument");
+		counts[0] = g->g.mask_passes;
+		counts[1] = g->g.scc_rounds;
+		counts[2] = g->g.host_syncs;
+	});
+}
+
 esl_status esl_graph_query(const esl_graph* g, int64_t sizes[7])
 {
 	return guarded(nullptr, [&] {

@@ -270,6 +270,8 @@ struct graph_t
 	std::vector<int32_t> h_offsets; // host copy of offsets (S + 1)
 
 	double stage_ms[2] = { 0, 0 };
+	// graph stage counters: passes over the flip masks, SCC rounds, host round trips
+	int64_t mask_passes = 0, scc_rounds = 0, host_syncs = 0;
 	solution_t sol;
 };
 

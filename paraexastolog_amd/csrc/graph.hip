@@ -48,15 +48,14 @@ __device__ __forceinline__ int high_bit(int j, int N) { return 6 + j < N ? 6 + j
 // pass whose predecessor changed nothing (prev[0] == 0) returns at once.
 __device__ __forceinline__ bool converged(const int32_t* prev) { return prev && prev[0] == 0; }
 
+// The per-vertex bodies below are called by every lane of a wave together
+// (they use cross-lane operations); v is the lane's vertex, valid when v < n.
+
 // ---- trimming: a vertex without an active in- or out-neighbour is a singleton SCC
 template <int HMAX>
-__global__ __launch_bounds__(gblock) void k_trim(int N, int64_t n, const uint32_t* __restrict__ mask,
-												 int32_t* __restrict__ scc, int32_t* __restrict__ changed,
-												 const int32_t* __restrict__ prev)
+__device__ __forceinline__ bool trim_vertex(int N, int64_t n, int64_t v, const uint32_t* __restrict__ mask,
+											int32_t* __restrict__ scc)
 {
-	if (converged(prev))
-		return;
-	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const uint32_t mv = valid ? mask[v] : 0u;
 	bool active = valid && scc[v] < 0;
@@ -104,10 +103,19 @@ __global__ __launch_bounds__(gblock) void k_trim(int N, int64_t n, const uint32_
 		}
 	}
 	if (trimmed)
-	{
 		scc[v] = (int32_t)v;
+	return trimmed;
+}
+
+template <int HMAX>
+__global__ __launch_bounds__(gblock) void k_trim(int N, int64_t n, const uint32_t* __restrict__ mask,
+												 int32_t* __restrict__ scc, int32_t* __restrict__ changed,
+												 const int32_t* __restrict__ prev)
+{
+	if (converged(prev))
+		return;
+	if (trim_vertex<HMAX>(N, n, (int64_t)blockIdx.x * gblock + threadIdx.x, mask, scc))
 		changed[0] = 1;
-	}
 }
 
 __global__ __launch_bounds__(gblock) void k_color_init(int64_t n, const int32_t* __restrict__ scc,
@@ -123,13 +131,9 @@ __global__ __launch_bounds__(gblock) void k_color_init(int64_t n, const int32_t*
 
 // ---- min-colour propagation along edges (u -> v: col[v] = min(col[v], col[u]))
 template <int HMAX>
-__global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32_t* __restrict__ mask,
-												  const int32_t* __restrict__ scc, int32_t* __restrict__ col,
-												  int32_t* __restrict__ changed, const int32_t* __restrict__ prev)
+__device__ __forceinline__ bool color_vertex(int N, int64_t n, int64_t v, const uint32_t* __restrict__ mask,
+											 const int32_t* __restrict__ scc, int32_t* __restrict__ col)
 {
-	if (converged(prev))
-		return;
-	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const bool active = valid && scc[v] < 0;
 	const uint32_t mv = valid ? mask[v] : 0u;
@@ -175,21 +179,29 @@ __global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32
 	if (active && c != c0)
 	{
 		col[v] = c;
-		changed[0] = 1;
+		return true;
 	}
+	return false;
+}
+
+template <int HMAX>
+__global__ __launch_bounds__(gblock) void k_color(int N, int64_t n, const uint32_t* __restrict__ mask,
+												  const int32_t* __restrict__ scc, int32_t* __restrict__ col,
+												  int32_t* __restrict__ changed, const int32_t* __restrict__ prev)
+{
+	if (converged(prev))
+		return;
+	if (color_vertex<HMAX>(N, n, (int64_t)blockIdx.x * gblock + threadIdx.x, mask, scc, col))
+		changed[0] = 1;
 }
 
 // ---- backward closure inside a colour class: v joins the SCC of its colour's
 // root if it has an edge into a member with the same colour.
 template <int HMAX>
-__global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32_t* __restrict__ mask,
-												  const int32_t* __restrict__ scc, const int32_t* __restrict__ col,
-												  uint8_t* __restrict__ inscc, int32_t* __restrict__ changed,
-												  const int32_t* __restrict__ prev)
+__device__ __forceinline__ bool close_vertex(int N, int64_t n, int64_t v, const uint32_t* __restrict__ mask,
+											 const int32_t* __restrict__ scc, const int32_t* __restrict__ col,
+											 uint8_t* __restrict__ inscc)
 {
-	if (converged(prev))
-		return;
-	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	const bool valid = v < n;
 	const bool active = valid && scc[v] < 0;
 	const uint32_t mv = valid ? mask[v] : 0u;
@@ -238,8 +250,21 @@ __global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32
 	if (in && !in0)
 	{
 		inscc[v] = 1;
-		changed[0] = 1;
+		return true;
 	}
+	return false;
+}
+
+template <int HMAX>
+__global__ __launch_bounds__(gblock) void k_close(int N, int64_t n, const uint32_t* __restrict__ mask,
+												  const int32_t* __restrict__ scc, const int32_t* __restrict__ col,
+												  uint8_t* __restrict__ inscc, int32_t* __restrict__ changed,
+												  const int32_t* __restrict__ prev)
+{
+	if (converged(prev))
+		return;
+	if (close_vertex<HMAX>(N, n, (int64_t)blockIdx.x * gblock + threadIdx.x, mask, scc, col, inscc))
+		changed[0] = 1;
 }
 
 __global__ __launch_bounds__(gblock) void k_assign(int64_t n, int32_t* __restrict__ scc,
@@ -263,14 +288,10 @@ __global__ __launch_bounds__(gblock) void k_assign(int64_t n, int32_t* __restric
 // ---- heights.  Level `cur` is enqueued in groups like the fixpoint passes;
 // once every root is labelled (total[0] == S) the kernels return at once.
 template <int HMAX>
-__global__ __launch_bounds__(gblock) void k_height_check(int N, int64_t n, const uint32_t* __restrict__ mask,
-														 const int32_t* __restrict__ scc,
-														 const int32_t* __restrict__ h, int32_t* __restrict__ blocked,
-														 int32_t cur, const int32_t* __restrict__ total, int32_t S)
+__device__ __forceinline__ void height_check_vertex(int N, int64_t n, int64_t v, const uint32_t* __restrict__ mask,
+													const int32_t* __restrict__ scc, const int32_t* __restrict__ h,
+													int32_t* __restrict__ blocked, int32_t cur)
 {
-	if (total[0] >= S)
-		return;
-	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
 	if (v >= n)
 		return;
 	const int32_t r = scc[v];
@@ -294,6 +315,30 @@ __global__ __launch_bounds__(gblock) void k_height_check(int N, int64_t n, const
 		blocked[r] = cur;
 }
 
+// 1 when v is a root still unlabelled and not blocked at level cur (then labelled cur)
+__device__ __forceinline__ int height_assign_vertex(int64_t n, int64_t v, const int32_t* __restrict__ scc,
+													int32_t* __restrict__ h, const int32_t* __restrict__ blocked,
+													int32_t cur)
+{
+	if (v < n && scc[v] == (int32_t)v && h[v] == 0 && blocked[v] != cur)
+	{
+		h[v] = cur;
+		return 1;
+	}
+	return 0;
+}
+
+template <int HMAX>
+__global__ __launch_bounds__(gblock) void k_height_check(int N, int64_t n, const uint32_t* __restrict__ mask,
+														 const int32_t* __restrict__ scc,
+														 const int32_t* __restrict__ h, int32_t* __restrict__ blocked,
+														 int32_t cur, const int32_t* __restrict__ total, int32_t S)
+{
+	if (total[0] >= S)
+		return;
+	height_check_vertex<HMAX>(N, n, (int64_t)blockIdx.x * gblock + threadIdx.x, mask, scc, h, blocked, cur);
+}
+
 __global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32_t* __restrict__ scc,
 														  int32_t* __restrict__ h,
 														  const int32_t* __restrict__ blocked, int32_t cur,
@@ -303,12 +348,7 @@ __global__ __launch_bounds__(gblock) void k_height_assign(int64_t n, const int32
 	if (total[0] >= S)
 		return;
 	const int64_t v = (int64_t)blockIdx.x * gblock + threadIdx.x;
-	int got = 0;
-	if (v < n && scc[v] == (int32_t)v && h[v] == 0 && blocked[v] != cur)
-	{
-		h[v] = cur;
-		got = 1;
-	}
+	const int got = height_assign_vertex(n, v, scc, h, blocked, cur);
 	const int w = __popcll(__ballot(got));
 	if (lane_id() == 0 && w)
 		atomicAdd(labelled, w);
@@ -413,12 +453,13 @@ struct pass_flags
 // first group has `hint` passes (what the same fixpoint needed last time, plus
 // one), later groups `group`; hint is updated to the passes used plus one.
 template <typename F>
-void fixpoint(ctx_t& c, int& hint, int group, int max_passes, const char* what, F&& pass)
+void fixpoint(ctx_t& c, graph_t& g, int& hint, int group, int max_passes, const char* what, F&& pass)
 {
 	constexpr int base = 8, slots = 16; // flag slots base .. base + slots - 1
 	int32_t* d = c.flags.get() + base;
 	for (int done = 0;;)
 	{
+		++g.host_syncs;
 		const int gsz = std::min(slots, std::max(2, done ? group : hint));
 		ESL_HIP(hipMemsetAsync(d, 0, gsz * sizeof(int32_t), c.stream));
 		for (int p = 0; p < gsz; ++p)
@@ -429,6 +470,7 @@ void fixpoint(ctx_t& c, int& hint, int group, int max_passes, const char* what, 
 			if (!c.h_flags[base + p])
 			{
 				hint = done + p + 2;
+				g.mask_passes += done + p + 1;
 				return;
 			}
 		done += gsz;
@@ -479,23 +521,25 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 	const int pass_group = 8;	   // fixpoint passes per host round trip, after the first
 	int hint[3] = { 8, 8, 8 }; // first-group sizes of the three fixpoints
 	int64_t prev_left = n + 1;
+	g.mask_passes = g.scc_rounds = g.host_syncs = 0;
 	for (int round = 0;; ++round)
 	{
+		++g.scc_rounds;
 		// algorithmic bytes of one pass over the flip masks: mask (4 B/state) and the
 		// state labels it reads and writes
-		fixpoint(c, hint[0], pass_group, max_passes, "trim", [&](int32_t* f, const int32_t* pf) {
+		fixpoint(c, g, hint[0], pass_group, max_passes, "trim", [&](int32_t* f, const int32_t* pf) {
 			timed_launch(c, "k_trim", 12.0 * n, [&] {
 				ESL_HMAX_DISPATCH(N, k_trim, <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), f, pf));
 			});
 		});
 		k_color_init<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get());
-		fixpoint(c, hint[1], pass_group, max_passes, "colouring", [&](int32_t* f, const int32_t* pf) {
+		fixpoint(c, g, hint[1], pass_group, max_passes, "colouring", [&](int32_t* f, const int32_t* pf) {
 			timed_launch(c, "k_color", 16.0 * n, [&] {
 				ESL_HMAX_DISPATCH(N, k_color,
 								  <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(), f, pf));
 			});
 		});
-		fixpoint(c, hint[2], pass_group, max_passes, "closure", [&](int32_t* f, const int32_t* pf) {
+		fixpoint(c, g, hint[2], pass_group, max_passes, "closure", [&](int32_t* f, const int32_t* pf) {
 			timed_launch(c, "k_close", 14.0 * n, [&] {
 				ESL_HMAX_DISPATCH(N, k_close,
 								  <<<nb, gblock, 0, c.stream>>>(N, n, g.mask.get(), g.scc.get(), col.get(),
@@ -507,6 +551,7 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 					 [&] { k_assign<<<nb, gblock, 0, c.stream>>>(n, g.scc.get(), col.get(), inscc.get(), fl.d + 1); });
 		ESL_HIP(hipGetLastError());
 		const int32_t left = fl.read(1);
+		++g.host_syncs;
 		if (!left)
 			break;
 		// every round removes at least the SCC of the smallest active vertex
@@ -559,9 +604,11 @@ void graph_reorganize(ctx_t& c, graph_t& g)
 		ESL_HIP(hipMemcpyAsync(c.h_flags + hbase, dl, level_group * sizeof(int32_t), hipMemcpyDeviceToHost,
 							   c.stream));
 		ESL_HIP(hipStreamSynchronize(c.stream));
+		++g.host_syncs;
 		for (int k = 0; k < level_group && !H; ++k)
 		{
 			const int32_t got = c.h_flags[hbase + k];
+			++g.mask_passes;
 			if (got == 0)
 				throw error(ESL_E_DEVICE, "condensation graph is not acyclic");
 			if (cur + 1 + k == 1)
