@@ -9,18 +9,21 @@
 // and multiplied with v_mfma_f32_32x32x16_bf16, accumulating in FP32.  Master
 // weights, activations and Adam moments are FP32.
 //
-// GEMM: a workgroup of 4 waves computes a 64 x 64 tile of C.  Each K stage of
-// 32 brings a 64 x 32 slice of A and a 32 x 64 slice of B from global memory
-// into registers (coalesced along whichever index is contiguous), rounds it to
-// BF16 once and stores it in LDS as rows of 32 k values (A by m, B by n); the
-// four waves then read their 32 x 32 sub-tiles' operands from LDS with 16-byte
-// reads and issue two MFMA 32x32x16 each.  The next stage's global loads are in
-// flight while the current stage is multiplied (LDS double buffer).  Operand
-// element (m, k) of A is A[m * sam + k * sak] and (k, n) of B is
-// B[k * sbk + n * sbn], so the same kernel serves X W, dZ W^T and X^T dZ.
-// Products with few output tiles and a long K (dZ W^T of the widest layer) are
-// split over gridDim.z; each split writes FP32 partial tiles that
-// k_gemm_reduce sums in split order (deterministic) before the epilogue.
+// GEMM: a workgroup of 4 waves computes a 64 x 64 tile of C.  Each K stage
+// (BK = 128, or 32 for products with K < 128) brings a 64 x BK slice of A and
+// a BK x 64 slice of B from global memory into registers (16-byte loads along
+// whichever index is contiguous when the operand allows, single floats
+// otherwise; interior tiles without bounds tests), rounds it to BF16 once and
+// stores it in LDS as rows of BK k values (A by m, B by n); the four waves then
+// read their 32 x 32 sub-tiles' operands from LDS with 16-byte reads and issue
+// BK / 16 MFMA 32x32x16 each.  The next stage's global loads are in flight
+// while the current stage is multiplied (LDS double buffer).  Operand element
+// (m, k) of A is A[m * sam + k * sak] and (k, n) of B is B[k * sbk + n * sbn],
+// so the same kernel serves X W, dZ W^T and X^T dZ.  Products with few output
+// tiles and a long K (dZ W^T of the widest layer) are split over gridDim.z;
+// each split writes FP32 partial tiles that k_gemm_reduce sums in split order
+// (deterministic) before the epilogue.  The epilogues (bias + GELU, bias,
+// times GELU'(Z)) are fused, and what they read is loaded before the K loop.
 // Register layout of the instruction (wave64): lane l holds
 // A[l % 32][8 (l / 32) + 0..7], B[8 (l / 32) + 0..7][l % 32] and C rows
 // 8 (i / 4) + 4 (l / 32) + (i % 4), column l % 32, for i = 0..15.
