@@ -215,3 +215,16 @@ def test_preconditioners_and_precisions(ctx, name, pre, prec):
     assert nz.tolist() == c["nonzero_indices"]
     np.testing.assert_allclose(v, c["nonzero_data"], rtol=0, atol=c["tol"])
     assert s.report["residual"] <= tol
+
+
+def test_graph_stage_counts(ctx):
+    # N=16 generated network: the counters of the graph stage are consistent with
+    # its result (every SCC round costs at least one trimming, colouring and
+    # closure pass and one round trip; every height level is a pass)
+    net, rates = synth_case("c2", 11)
+    pm = model_builder().parse_model(net.bnet())
+    t, g = build(ctx, pm, rates[0])
+    c = g.stage_counts()
+    assert c["scc_rounds"] >= 1 and c["host_round_trips"] >= 4 * c["scc_rounds"]
+    assert c["mask_passes"] >= 3 * c["scc_rounds"] + g.max_height
+
