@@ -218,5 +218,29 @@ struct persistent_solution {              // persistent_solution.h:27-37 (.symb,
         return esl_symb::compatible_zero_rates(d, r);
     }
     static bool are_same(const persistent_data& d, const std::vector<double>& r) { return esl_symb::same_rates(d, r); }
+    // check_are_equal (persistent_solution.cu:247-269): every stored array equals
+    // the solver's, values compared as the file holds them (FP32)
+    static bool check_are_equal(const solver& s, const persistent_data& d) {
+        if (!s.t || !s.gr) throw std::runtime_error("check_are_equal: the solver was not built from a table and a graph");
+        const auto same = [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) { return a == b; };
+        const auto same_f = [](const std::vector<double>& a, const std::vector<float>& b) {
+            if (a.size() != b.size()) return false;
+            for (size_t i = 0; i < a.size(); ++i)
+                if ((float)a[i] != b[i]) return false;
+            return true;
+        };
+        const auto& off = s.gr->sccs_offsets;
+        const size_t T = (size_t)s.gr->terminals_count;
+        if (off.size() < T + 1) return false;
+        bool eq = same(s.t->rows, d.rows) && same(s.t->cols, d.cols) && same(s.t->indptr, d.indptr);
+        eq = eq && same(s.gr->reordered_vertices, d.ordered_vertices);
+        eq = eq && same(std::vector<int32_t>(off.begin(), off.begin() + T + 1), d.terminals_offsets);
+        eq = eq && same(std::vector<int32_t>(off.begin() + T, off.end()), d.nonterminals_offsets);
+        eq = eq && same_f(s.rates, d.rates);
+        eq = eq && same(s.solution_term.indptr, d.term_indptr) && same(s.solution_term.indices, d.term_indices);
+        eq = eq && same_f(s.solution_term.data, d.term_data);
+        eq = eq && same(s.solution_nonterm.indptr, d.nonterm_indptr) && same(s.solution_nonterm.indices, d.nonterm_indices);
+        return eq && same_f(s.solution_nonterm.data, d.nonterm_data);
+    }
 };
 } // namespace compat

@@ -103,6 +103,7 @@ int main(int argc, char** argv) {
 
     compat::persistent_solution::serialize(argv[2], s, true);
     auto persisted = compat::persistent_solution::deserialize(argv[2], true);
+    std::printf("equal %d\n", (int)compat::persistent_solution::check_are_equal(s, persisted));
     compat::solver same(context, persisted, r, st);
     same.solve();
     double d_same = 0;
@@ -169,3 +170,99 @@ def test_compat_solver_toy_like_reference_test(tmp_path):
     r = rec["recompute"]
     assert r[0] == "0" and r[1] == "1" and float(r[3]) <= 1e-12 and float(r[4]) <= 1e-12
     assert "mixed vectors rejected" in out
+    assert rec["equal"] == ["1"]
+
+
+# cuda/test/test.cu serializer.zanudo (:594-640): N=23, uniform rates, five nodes
+# fixed; the solved solver is written and read back with and without the
+# inverse flag, and every stored array must equal the solver's
+PROG_SERIAL = r"""
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include "esl_compat.hpp"
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+int main(int argc, char** argv) {
+    auto t0 = std::chrono::steady_clock::now();
+    compat::model_builder builder;
+    auto model = builder.construct_model(argv[1]);
+    compat::cu_context context;
+    compat::transition_rates r(model);
+    r.generate_uniform();
+    compat::transition_table table(context, model, r.rates);
+    table.construct_table();
+    compat::transition_graph g(context, table.rows, table.cols, table.indptr);
+    g.reorganize_vertices();
+    compat::initial_state st(model, {"Alpelisib", "Everolimus", "PIM", "Proliferation", "Apoptosis"},
+                             {false, true, false, false, false}, 1.f);
+    compat::solver s(context, table, g, r, st);
+    s.solve();
+    std::fprintf(stderr, "solved in %.0f ms: %d terminals, %lld entries in solution_nonterm\n", ms_since(t0),
+                 g.terminals_count, (long long)s.solution_nonterm.nnz());
+    std::printf("terminals %d %lld %lld\n", g.terminals_count, (long long)s.solution_term.nnz(),
+                (long long)s.solution_nonterm.nnz());
+    std::printf("final");
+    for (size_t i = 0; i < s.final_state.size(); ++i)
+        if (s.final_state[i] != 0) std::printf(" %zu:%.17g", i, s.final_state[i]);
+    std::printf("\n");
+    for (int i = 0; i < 2; ++i) {
+        const bool no_inverse = i == 0;
+        auto t1 = std::chrono::steady_clock::now();
+        compat::persistent_solution::serialize(argv[2], s, no_inverse);
+        const double ser = ms_since(t1);
+        t1 = std::chrono::steady_clock::now();
+        auto d = compat::persistent_solution::deserialize(argv[2], no_inverse);
+        const double des = ms_since(t1);
+        std::fprintf(stderr, "serialize %.0f ms, deserialize %.0f ms (no_inverse %d)\n", ser, des, (int)no_inverse);
+        std::printf("equal %d %d\n", (int)no_inverse, (int)compat::persistent_solution::check_are_equal(s, d));
+        // the stored solution reused: R (L x0) from the FP32 arrays (solver.cu:302-309)
+        compat::solver again(context, d, r, st);
+        again.solve();
+        double l1 = 0;
+        for (size_t k = 0; k < s.final_state.size(); ++k) l1 += std::fabs(again.final_state[k] - s.final_state[k]);
+        std::printf("reuse %d %d %.3g\n", (int)no_inverse, (int)again.recompute_needed, l1);
+    }
+    return 0;
+}
+"""
+
+
+def _build_serial(tmp_path):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    src = tmp_path / "z.cpp"
+    src.write_text(PROG_SERIAL)
+    exe = tmp_path / "z"
+    libdir = os.path.dirname(LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-lesl", f"-Wl,-rpath,{libdir}"], check=True)
+    return exe
+
+
+def test_compat_serializer_body_compiles(tmp_path):
+    _build_serial(tmp_path)
+
+
+@pytest.mark.gpu
+def test_compat_serializer_zanudo(tmp_path):
+    from .helpers import golden
+
+    c = golden()["solver.zanudo"]
+    exe = _build_serial(tmp_path)
+    out = subprocess.run([str(exe), model_path("zanudo_expanded.bnet"), str(tmp_path / "z.symb")],
+                         stdout=subprocess.PIPE, text=True, check=True, timeout=600).stdout.splitlines()
+    rec = {}
+    for ln in out:
+        k, *v = ln.split()
+        rec.setdefault(k, []).append(v)
+    # 2^23 singleton SCCs, 57 of them terminal, and 11,324,352 (state, reachable
+    # terminal) pairs: the oracle's graph with a reachability sweep by height
+    assert rec["terminals"][0] == ["57", "57", "11324352"]
+    fin = dict(kv.split(":") for kv in rec["final"][0])
+    assert [int(k) for k in fin] == c["nonzero_indices"]
+    assert [float(v) for v in fin.values()] == pytest.approx(c["nonzero_data"], abs=c["tol"])
+    assert rec["equal"] == [["1", "1"], ["0", "1"]]
+    for no_inverse, recompute, l1 in rec["reuse"]:
+        assert recompute == "0" and float(l1) <= 1e-6  # FP32 storage of R and L

@@ -198,6 +198,32 @@ def test_solution_export_vs_oracle(ctx, name):
     assert l1(s.final_state, ref.final_state) <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["mammalian_cc.bnet", "krasmodel15vars.bnet"])
+def test_solution_export_at_size(ctx, name):
+    # absorption probabilities for every transient state: mammal (N=13, 3
+    # terminals, 3,888 states reaching two or more) stays on the per-set
+    # solver; kras (N=15, 40 terminals, 20,944 such states) takes the
+    # device-wide one with one right-hand side per terminal
+    om = read_bnet(model_path(name))
+    rates = np.exp(np.random.default_rng(len(name) + 1).uniform(np.log(0.1), np.log(10), 2 * om.n_nodes))
+    pm = model_builder().construct_model(model_path(name))
+    t, g = build(ctx, pm, rates)
+    s = solver(ctx, t, g, rates, initial_state(pm.nodes), full=1)
+    s.solve()
+    n, ip, rows, cols, graph = oracle_graph(om, rates)
+    ref = P.solve(n, ip, rows, rates, P.initial_state(om), graph, full=True)
+    R, L = s.solution_term, s.solution_nonterm
+    np.testing.assert_array_equal(R.indptr, ref.term_indptr)
+    np.testing.assert_array_equal(R.indices, ref.term_indices)
+    np.testing.assert_allclose(R.data, ref.term_data, rtol=1e-10, atol=1e-13)
+    np.testing.assert_array_equal(L.indptr, ref.nonterm_indptr)
+    np.testing.assert_array_equal(L.indices, ref.nonterm_indices)
+    np.testing.assert_allclose(L.data, ref.nonterm_data, rtol=1e-9, atol=1e-12)
+    # every state's absorption probabilities sum to one
+    np.testing.assert_allclose(np.bincount(L.indices, weights=L.data, minlength=n), 1.0, rtol=0, atol=1e-11)
+    assert l1(s.final_state, ref.final_state) <= 1e-12
+
+
 @pytest.mark.parametrize("pre", [1, 2])
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("name", ["solver.kras", "solver.mammal", "symbolic_with_refactor.cohen"])
