@@ -266,3 +266,107 @@ def test_compat_serializer_zanudo(tmp_path):
     assert rec["equal"] == [["1", "1"], ["0", "1"]]
     for no_inverse, recompute, l1 in rec["reuse"]:
         assert recompute == "0" and float(l1) <= 1e-6  # FP32 storage of R and L
+
+
+# cuda/test/test.cu symbolic_no_refactor.cohen (:701-778) and
+# symbolic_with_refactor.cohen (:842-925): N=20, the first solve written to a
+# .symb file and read back with and without the inverse flag, then solved from
+# the file with the same rates (stored solution reused, new initial state) and
+# with rates of the same zero pattern (table and solve redone on the stored
+# graph stage)
+PROG_COHEN = r"""
+#include <cstdio>
+#include "esl_compat.hpp"
+static void show(const char* tag, int no_inverse, const compat::solver& s) {
+    std::printf("%s %d %d", tag, no_inverse, (int)s.recompute_needed);
+    for (size_t i = 0; i < s.final_state.size(); ++i)
+        if ((float)s.final_state[i] != 0.0f) std::printf(" %zu:%.9g", i, s.final_state[i]);
+    std::printf("\n");
+}
+int main(int argc, char** argv) {
+    compat::model_builder builder;
+    auto model = builder.construct_model(argv[1]);
+    const std::string file = argv[2];
+    compat::cu_context context;
+    const std::vector<std::string> fixed = {"ECMicroenv", "DNAdamage", "Metastasis", "Migration", "Invasion",
+                                            "EMT", "Apoptosis", "Notch_pthw", "p53"};
+    const std::vector<bool> vals = {true, true, false, false, false, false, false, true, false};
+    compat::transition_rates r(model);
+    r.generate_uniform({}, {});
+    compat::transition_rates r2(model);
+    r2.generate_uniform({{"DNAdamage", 2.f}, {"ECMicroenv", 3.f}, {"TGFb_pthw", 4.f}, {"Apoptosis", 300.f},
+                         {"Ecadh", 1000.f}},
+                        {{"TGFb_pthw", 933.f}, {"Apoptosis", 544.f}, {"Ecadh", 3.f}});
+    compat::initial_state st2(model, fixed, vals, 1.f);
+    {   // symbolic_no_refactor: first solve from every state
+        compat::transition_table table(context, model, r.rates);
+        table.construct_table();
+        compat::transition_graph g(context, table.rows, table.cols, table.indptr);
+        g.reorganize_vertices();
+        compat::solver s(context, table, g, r, compat::initial_state(model));
+        s.solve();
+        for (int i = 0; i < 2; ++i) {
+            compat::persistent_solution::serialize(file, s, i == 0);
+            auto d = compat::persistent_solution::deserialize(file, i == 0);
+            compat::solver s2(context, d, r, st2);
+            s2.solve();
+            show("no_refactor", i == 0, s2);
+        }
+    }
+    {   // symbolic_with_refactor: first solve from the fixed initial state
+        compat::transition_table table(context, model, r.rates);
+        table.construct_table();
+        compat::transition_graph g(context, table.rows, table.cols, table.indptr);
+        g.reorganize_vertices();
+        compat::solver s(context, table, g, r, st2);
+        s.solve();
+        for (int i = 0; i < 2; ++i) {
+            compat::persistent_solution::serialize(file, s, i == 0);
+            auto d = compat::persistent_solution::deserialize(file, i == 0);
+            std::printf("compatible %d\n", (int)compat::persistent_solution::has_compatible_zero_rates(d, r2.rates));
+            compat::solver s2(context, d, r2, st2);
+            s2.solve();
+            show("with_refactor", i == 0, s2);
+        }
+    }
+    return 0;
+}
+"""
+
+
+def _build_cohen(tmp_path):
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    src = tmp_path / "c.cpp"
+    src.write_text(PROG_COHEN)
+    exe = tmp_path / "c"
+    libdir = os.path.dirname(LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-lesl", f"-Wl,-rpath,{libdir}"], check=True)
+    return exe
+
+
+def test_compat_symbolic_cohen_body_compiles(tmp_path):
+    _build_cohen(tmp_path)
+
+
+@pytest.mark.gpu
+def test_compat_symbolic_cohen(tmp_path):
+    from .helpers import golden
+
+    g = golden()
+    exe = _build_cohen(tmp_path)
+    out = subprocess.run([str(exe), model_path("EMT_cohen_ModNet.bnet"), str(tmp_path / "f.symb")],
+                         stdout=subprocess.PIPE, text=True, check=True, timeout=300).stdout.splitlines()
+    rows = [ln.split() for ln in out]
+    assert [r for r in rows if r[0] == "compatible"] == [["compatible", "1"]] * 2
+    for tag, case, recompute in (("no_refactor", "symbolic_no_refactor.cohen", "0"),
+                                 ("with_refactor", "symbolic_with_refactor.cohen", "1")):
+        c = g[case]
+        got = [r for r in rows if r[0] == tag]
+        assert [r[1] for r in got] == ["1", "0"]  # no_inverse true, then false
+        for r in got:
+            assert r[2] == recompute
+            fin = dict(kv.split(":") for kv in r[3:])
+            assert [int(k) for k in fin] == c["nonzero_indices"]
+            assert [float(v) for v in fin.values()] == pytest.approx(c["nonzero_data"], abs=c["tol"])
