@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--schedule", type=int, default=None,
                     help="pin a schedule instead of autotuning (0 per-set, 1 per-set global, 2 device-wide, "
                          "3 per-set LDS with the basis in global memory, 4/5 per-set at 3/2 workgroups per CU)")
+    ap.add_argument("--restart", type=int, default=32, help="GMRES restart length of a pinned --schedule")
     ap.add_argument("--workload", default="n20", choices=["n20", "n16", "uneven", "n22", "warm", "seeds"],
                     help="n20 (default) is the headline metric; the others print their own JSON line")
     ap.add_argument("--epochs", type=int, default=100, help="training epochs of the warm workload")
@@ -112,10 +113,10 @@ def stage_bandwidth(stats, n, m, counts=None):
     return out
 
 
-def traffic_record(kernel, workload):
+def traffic_record(kernel, workload, restart=None):
     """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE passes, scaled by the same counters on a 1 GiB copy), when a
-    record for this kernel and workload exists under profiles/."""
+    record for this kernel, workload and restart length exists under profiles/."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -123,7 +124,7 @@ def traffic_record(kernel, workload):
     except (OSError, ValueError):
         return None
     for r in recs:
-        if r.get("kernel") == kernel and r.get("workload") == workload:
+        if r.get("kernel") == kernel and r.get("workload") == workload and r.get("restart") in (None, restart):
             return r
     return None
 
@@ -402,9 +403,9 @@ def main():
     schedule = None
     restart = 32
     if a.schedule is not None:
-        check(L.esl_plan_set_schedule(pl, a.schedule, 32))
+        check(L.esl_plan_set_schedule(pl, a.schedule, a.restart))
         schedule = {"path": SCHEDULE_NAMES[a.schedule],
-                    "restart": 32, "pinned": True}
+                    "restart": a.restart, "pinned": True}
         restart = 0
     elif a.autotune:
         o = solve_opts(precision=a.precision, tol=tol, inner_rtol=a.inner_rtol)
@@ -565,7 +566,7 @@ def main():
         "cpu_baseline": cpu,
         "latency_roof": latency,
     }
-    tr = traffic_record(dom[0], f"N=20 seed {a.seed} sets {B}")
+    tr = traffic_record(dom[0], f"N=20 seed {a.seed} sets {B}", (schedule or {}).get("restart"))
     if tr is not None:
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
         out["roofline"]["traffic_source"] = tr["source"]
