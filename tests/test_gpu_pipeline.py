@@ -111,6 +111,7 @@ def test_table_and_graph_bit_exact(ctx, name):
     np.testing.assert_array_equal(g.reordered_vertices, reord)
     np.testing.assert_array_equal(g.sccs_offsets, off)
     assert g.terminals_count == T
+    assert g.max_height == int(P.scc_heights(n, ip, rows, ids)[1].max())
 
 
 def test_zero_rate_edges_removed(ctx):
