@@ -155,6 +155,9 @@ struct solver {                           // solver.h:10-67
         if (persisted) {                            // stored graph stage, new rates of the same zero pattern
             if (!esl_symb::compatible_zero_rates(*persisted, rates))
                 throw std::runtime_error("rates are not compatible with the stored symbolic data");
+            if (persisted->ordered_vertices.size() != x0.size() || persisted->terminals_offsets.empty() ||
+                persisted->nonterminals_offsets.empty())
+                throw std::runtime_error("persistent data file is corrupted");
             ok(esl_table_build(ctx.h, model.h, rates.data(), &own), ctx.h);
             std::vector<int32_t> offs(persisted->terminals_offsets);
             offs.insert(offs.end(), persisted->nonterminals_offsets.begin() + 1, persisted->nonterminals_offsets.end());

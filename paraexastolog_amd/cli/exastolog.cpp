@@ -404,8 +404,8 @@ int main(int argc, char** argv)
 					ok(esl_graph_query(r.g, sz));
 					std::vector<int32_t> ip(sz[0] + 1), rw(sz[1]);
 					ok(esl_table_export(r.ctx, r.g, ip.data(), rw.data(), nullptr), r.ctx);
-					if (ip == d.indptr && rw == d.rows && !d.terminals_offsets.empty() &&
-						!d.nonterminals_offsets.empty())
+					if (ip == d.indptr && rw == d.rows && d.ordered_vertices.size() == (size_t)sz[0] &&
+						!d.terminals_offsets.empty() && !d.nonterminals_offsets.empty())
 					{
 						std::vector<int32_t> offsets(d.terminals_offsets);
 						offsets.insert(offsets.end(), d.nonterminals_offsets.begin() + 1, d.nonterminals_offsets.end());
