@@ -165,6 +165,36 @@ def test_synthetic_vs_oracle(ctx, cfg, seed):
         assert l1(x, ref) <= 1e-12
 
 
+# database-small models whose oracle solve (reachability-pruned, uniform initial
+# state) takes under a second; on the others (N = 14-19) it takes minutes
+DB_ORACLE = ["007.bnet", "031.bnet", "088.bnet", "109.bnet", "110.bnet", "158.bnet", "198.bnet", "208.bnet"]
+
+
+@pytest.mark.parametrize("name", SMALL_DB)
+def test_database_small_stationary(ctx, name):
+    # every database-small model, log-uniform rates, the uniform initial state:
+    # FP64 and FP32-with-refinement solves agree, conserve mass and meet the
+    # residual; where the oracle is fast, both match it
+    om = read_bnet(model_path("database-small/" + name))
+    rates = np.exp(np.random.default_rng(len(name) + 7).uniform(np.log(0.1), np.log(10), 2 * om.n_nodes))
+    pm = model_builder().construct_model(model_path("database-small/" + name))
+    t, g = build(ctx, pm, rates)
+    xs = []
+    for precision in (0, 1):
+        s = solver(ctx, t, g, rates, initial_state(pm.nodes), precision=precision, tol=1e-12)
+        s.solve()
+        x = s.final_state
+        # mass is conserved to what the 1e-12 relative residual fixes (2e-11 on
+        # 074, whose 262,144 states all carry initial mass)
+        assert abs(x.sum() - 1.0) < 1e-10 and x.min() >= -1e-14
+        assert s.report["residual"] <= 1e-12
+        xs.append(x)
+    assert l1(xs[0], xs[1]) <= 1e-10
+    if name in DB_ORACLE:
+        ref = P.Pipeline(om, rates, P.initial_state(om), full=False).final_state
+        assert l1(xs[0], ref) <= 1e-12 and l1(xs[1], ref) <= 1e-10
+
+
 @pytest.mark.parametrize("name", ["solver.toy", "solver.toy2", "solver.toy3"])
 def test_solution_export_goldens(ctx, name):
     # solution_term (CSC) and solution_nonterm (CSR) in the reference layout (test.cu:101-111)
