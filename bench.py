@@ -710,24 +710,25 @@ def workload_uneven(a, ctx):
 
 
 def workload_seeds(a, ctx):
-    """Representative instances: N=20 seeds 0..7 (the metric's network family)
-    and N=22 seeds 0..3 (the mixed-precision family).  Per seed: the plan's
-    unknowns, the schedule, batched FP32 + FP64 refinement sets/s (1024 sets when
-    the system fits the per-set kernels, autotuned; 64 sets device-wide
-    otherwise), the plain per-solve FP64 rate on the same plan, the speedup, and
-    the L1 distance of batched set 0 to its plain FP64 solve."""
+    """Representative instances: N=20 seeds 0..7 (the metric's network family),
+    N=22 seeds 0..3 (the mixed-precision family) and N=16 seeds 0..3 (the batch
+    family).  Per seed: the plan's unknowns, the schedule, batched FP32 + FP64
+    refinement sets/s (1024 sets when the system fits the per-set kernels,
+    autotuned; device-wide otherwise, 256 sets at N=16 and 64 above), the plain
+    per-solve FP64 rate on the same plan, the speedup, and the L1 distance of
+    batched set 0 to its plain FP64 solve."""
     from paraexastolog_amd import synth
     from paraexastolog_amd._lib import check, lib
     from paraexastolog_amd.surrogate import DeviceArray
 
     L = lib()
     recs = []
-    for cfg, seeds in (("c1", range(8)), ("c3", range(4))):
+    for cfg, seeds in (("c1", range(8)), ("c3", range(4)), ("c2", range(4))):
         for seed in seeds:
             net, rng = synth.config(cfg, seed)
             tp = _Topo(ctx, net, synth.rate_sets(rng, net.N, 1)[0])
             nU = tp.info["unknowns"]
-            B = 1024 if nU <= 4096 else 64
+            B = 1024 if nU <= 4096 else (256 if net.N <= 16 else 64)
             net2, rng2 = synth.config(cfg, seed)
             rates = synth.rate_sets(rng2, net2.N, B)
             dr = DeviceArray.from_host(ctx, rates)
@@ -761,10 +762,10 @@ def workload_seeds(a, ctx):
             print(json.dumps(rec), file=sys.stderr, flush=True)
             del tp, dr, df, d1
     out = {"metric": "rate sets/s per seed (batched FP32 + FP64 refinement) and speedup vs plain per-solve FP64",
-           "workload": "N=20 (K in {2,3}, 5 fixed nodes) seeds 0..7 and N=22 (K=3 + 3 spliced 3-cycles) seeds 0..3, "
-                       "generator v1; tol 1e-12 batched, 1e-13 plain",
+           "workload": "N=20 (K in {2,3}, 5 fixed nodes) seeds 0..7, N=22 (K=3 + 3 spliced 3-cycles) seeds 0..3 "
+                       "and N=16 (K=2) seeds 0..3, generator v1; tol 1e-12 batched, 1e-13 plain",
            "seeds": recs}
-    for N in (20, 22):
+    for N in (20, 22, 16):
         r = [x for x in recs if x["N"] == N]
         out[f"median_N{N}"] = {"sets_per_s": float(np.median([x["sets_per_s"] for x in r])),
                                "speedup_vs_plain_fp64": float(np.median([x["speedup_vs_plain_fp64"] for x in r])),
