@@ -195,6 +195,52 @@ def test_database_small_stationary(ctx, name):
         assert l1(xs[0], ref) <= 1e-12 and l1(xs[1], ref) <= 1e-10
 
 
+# edge-case networks: one node kept (two terminal states), one node negated
+# (one SCC of two states), every state terminal, one rule constant, and all
+# rates of one node zero
+EDGE_MODELS = {
+    "keep1": "targets, factors\nA, A\n",
+    "flip1": "targets, factors\nA, !A\n",
+    "all_terminal": "targets, factors\nA, A\nB, B\nC, C\n",
+    "constant": "targets, factors\nA, B | !B\nB, A & B\nC, !A\n",
+    "cycle3": "targets, factors\nA, C\nB, A\nC, !B\nD, D & A\n",
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE_MODELS))
+@pytest.mark.parametrize("zero_node", [None, 0])
+def test_edge_case_networks(ctx, name, zero_node):
+    text = EDGE_MODELS[name]
+    om = read_bnet_text(text)
+    rates = np.exp(np.random.default_rng(len(name)).uniform(np.log(0.1), np.log(10), 2 * om.n_nodes))
+    if zero_node is not None:
+        rates[2 * zero_node:2 * zero_node + 2] = 0.0
+    pm = model_builder().parse_model(text)
+    t, g = build(ctx, pm, rates)
+    n, ip, rows, cols, (reord, off, T, ids) = oracle_graph(om, rates)
+    np.testing.assert_array_equal(t.indptr, ip)
+    np.testing.assert_array_equal(t.rows, rows)
+    np.testing.assert_array_equal(g.reordered_vertices, reord)
+    np.testing.assert_array_equal(g.sccs_offsets, off)
+    assert g.terminals_count == T
+    for full in (0, 1):
+        s = solver(ctx, t, g, rates, initial_state(pm.nodes), full=full)
+        s.solve()
+        ref = P.solve(n, ip, rows, rates, P.initial_state(om), (reord, off, T, ids), full=True)
+        assert l1(s.final_state, ref.final_state) <= 1e-13
+        if name == "flip1" and zero_node is None:
+            # two states exchanging at rates up (0 -> 1) and down (1 -> 0):
+            # pi = (down, up) / (up + down)
+            up, down = rates[0], rates[1]
+            np.testing.assert_allclose(s.final_state, [down / (up + down), up / (up + down)], rtol=0, atol=1e-15)
+        if name in ("all_terminal", "keep1") or (name == "flip1" and zero_node == 0):
+            np.testing.assert_array_equal(s.final_state, np.full(n, 1.0 / n))  # no transitions: x0 stays
+        if full:
+            np.testing.assert_array_equal(s.solution_nonterm.indptr, ref.nonterm_indptr)
+            np.testing.assert_array_equal(s.solution_nonterm.indices, ref.nonterm_indices)
+            np.testing.assert_allclose(s.solution_nonterm.data, ref.nonterm_data, rtol=1e-12, atol=1e-14)
+
+
 @pytest.mark.parametrize("name", ["solver.toy", "solver.toy2", "solver.toy3"])
 def test_solution_export_goldens(ctx, name):
     # solution_term (CSC) and solution_nonterm (CSR) in the reference layout (test.cu:101-111)
