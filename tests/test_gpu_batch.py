@@ -45,6 +45,34 @@ def test_plan_batch_matches_single_solves(ctx, cfg, seed, precision):
     assert rep["residual"] <= 1e-12
 
 
+def test_large_batches(ctx):
+    # 2,500 sets on the per-set kernels (more workgroups than slots) and 1,100 on
+    # the device-wide solver (two chunks of at most 1,024 sets): the chunk and
+    # set offsets agree with single solves and with each other
+    B = 2500
+    net, rates, m, t, g, x0 = setup_net(ctx, "c0", 1, B)
+    n = 1 << net.N
+    dx0 = DeviceArray.from_host(ctx, x0)
+    dr = DeviceArray.from_host(ctx, rates)
+    dout = DeviceArray(ctx, (B, n), np.float64)
+    plan = solve_plan(ctx, t, dx0.ptr)
+    assert plan.info()["unknowns"] > 0
+    rep = plan.solve(B, dr.ptr, dout.ptr, precision=1, tol=1e-12, graph=1)
+    xb = dout.download()
+    assert rep["residual"] <= 1e-12
+    assert np.abs(xb.sum(axis=1) - 1).max() < 1e-11
+    for k in (0, 1023, 1024, 2047, B - 1):
+        s = solver(ctx, t, g, rates[k], x0)
+        s.solve()
+        assert np.abs(xb[k] - s.final_state).sum() <= 1e-10
+    Bw = 1100
+    dw = DeviceArray(ctx, (Bw, n), np.float64)
+    plan.set_schedule(2, 32)
+    rep = plan.solve(Bw, dr.ptr, dw.ptr, precision=1, tol=1e-12, restart=0)
+    assert rep["residual"] <= 1e-12
+    assert np.abs(dw.download() - xb[:Bw]).sum(axis=1).max() <= 1e-10
+
+
 def test_plan_matches_oracle_c0(ctx):
     B = 4
     net, rates, m, t, g, x0 = setup_net(ctx, "c0", 3, B)
