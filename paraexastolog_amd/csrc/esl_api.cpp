@@ -460,14 +460,12 @@ esl_status esl_solve_batch_device(esl_ctx* ctx, const esl_graph* g, int n_sets, 
 		if (opts)
 			o = *opts;
 		require(o.precision == 0 || o.precision == 1, ESL_E_ARG, "precision must be 0 or 1");
+		require(!o.full || n_sets == 1, ESL_E_ARG, "opts.full needs a single rate set");
 		esl_solve_report rep{};
 		esl::solve_args_t a{ d_rates, d_x0, d_final, n_sets, o };
 		esl::solve_run(ctx->c, const_cast<esl::graph_t&>(g->g), a, rep);
 		if (o.full)
-		{
-			require(n_sets == 1, ESL_E_ARG, "opts.full needs a single rate set");
 			esl::solution_full(ctx->c, const_cast<esl::graph_t&>(g->g), d_rates, o);
-		}
 		if (report)
 			*report = rep;
 	});
