@@ -124,6 +124,8 @@ esl_status esl_ctx_create(int device, esl_ctx** out)
 			ESL_HIP(hipEventCreate(&e));
 		ctx->c.flags.ensure(64);
 		ESL_HIP(hipHostMalloc((void**)&ctx->c.h_flags, 64 * sizeof(int32_t), hipHostMallocDefault));
+		ctx->c.h_stage_bytes = esl::stage_bytes;
+		ESL_HIP(hipHostMalloc((void**)&ctx->c.h_stage, ctx->c.h_stage_bytes, hipHostMallocDefault));
 		esl::noalloc_init(ctx->c);
 		int wclk = 0;
 		if (hipDeviceGetAttribute(&wclk, hipDeviceAttributeWallClockRate, device) == hipSuccess && wclk > 0)
@@ -144,6 +146,8 @@ void esl_ctx_destroy(esl_ctx* ctx)
 			(void)hipEventDestroy(e);
 	if (ctx->c.h_flags)
 		(void)hipHostFree(ctx->c.h_flags);
+	if (ctx->c.h_stage)
+		(void)hipHostFree(ctx->c.h_stage);
 	ctx->c.flags.reset();
 	ctx->c.scratch_a.reset();
 	ctx->c.scratch_b.reset();

@@ -164,6 +164,10 @@ struct ctx_t
 	dbuf<uint8_t> scratch_a, scratch_b;
 	dbuf<int32_t> flags;	// small device counters / flags
 	int32_t* h_flags = nullptr; // pinned mirror
+	// pinned staging of the per-set outputs of a launch, read back in one copy
+	// (allocated with the context: calls that may not allocate use it too)
+	unsigned char* h_stage = nullptr;
+	size_t h_stage_bytes = 0;
 	prof_t prof;
 	// forked streams (default priority) and fork/join events for plans solved concurrently
 	std::vector<hipStream_t> aux;
@@ -304,6 +308,9 @@ void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final,
 void plan_set_schedule(void* p, int path, int restart);
 void plan_set_results(const void* p, int B, double* res, int32_t* its, int32_t* method);
 void plan_set_times(const void* p, int B, double wall_khz, double* start_ms, double* end_ms);
+// pinned staging buffer of a context for per-set outputs (32 B per set with the
+// clocks: batches of up to 8192 sets in one copy)
+constexpr size_t stage_bytes = 256 * 1024;
 // the plan solve whose buffers are slices of the static scratch (no allocation)
 void noalloc_init(ctx_t& c); // once per device, with that device current
 void plan_solve_noalloc(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o,

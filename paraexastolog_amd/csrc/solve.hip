@@ -3816,15 +3816,50 @@ std::string sci(double v)
 
 // per-set results of the last launch: max / sums over the sets, and the host
 // copies of the per-set residuals, iteration and cycle counts
+// The per-set outputs of a launch (residuals, iterations, cycles and, when
+// asked, the start / end clocks) in one staged copy: the arrays go back to back
+// into the context's pinned buffer and the host waits once, where a copy per
+// array into pageable memory costs a host round trip each.  A batch too large
+// for the buffer is read one array at a time.
+void read_set_outputs(ctx_t& c, const pss_view_t& sc, int B, std::vector<double>& res, std::vector<int32_t>& its,
+					  std::vector<int32_t>& cyc, std::vector<uint64_t>* tspan)
+{
+	const size_t nb = (size_t)std::max(B, 0);
+	const size_t need = nb * (8 + (tspan ? 16 : 0) + 4 + 4);
+	if (!c.h_stage || need > c.h_stage_bytes || nb == 0)
+	{
+		read_host(c, res, sc.res, nb);
+		read_host(c, its, sc.its, nb);
+		read_host(c, cyc, sc.cyc, nb);
+		if (tspan)
+			read_host(c, *tspan, sc.tspan, 2 * nb);
+		return;
+	}
+	double* h_res = reinterpret_cast<double*>(c.h_stage);
+	uint64_t* h_ts = reinterpret_cast<uint64_t*>(c.h_stage + 8 * nb);
+	int32_t* h_its = reinterpret_cast<int32_t*>(c.h_stage + 8 * nb + (tspan ? 16 * nb : 0));
+	int32_t* h_cyc = h_its + nb;
+	ESL_HIP(hipMemcpyAsync(h_res, sc.res, 8 * nb, hipMemcpyDeviceToHost, c.stream));
+	if (tspan)
+		ESL_HIP(hipMemcpyAsync(h_ts, sc.tspan, 16 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipMemcpyAsync(h_its, sc.its, 4 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipMemcpyAsync(h_cyc, sc.cyc, 4 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	res.assign(h_res, h_res + nb);
+	its.assign(h_its, h_its + nb);
+	cyc.assign(h_cyc, h_cyc + nb);
+	if (tspan)
+		tspan->assign(h_ts, h_ts + 2 * nb);
+}
+
 krylov_result_t collect_sets(ctx_t& c, const pss_view_t& sc, int B, double tol, bool check,
-							 std::vector<double>* h_res_out = nullptr, std::vector<int32_t>* h_its_out = nullptr)
+							 std::vector<double>* h_res_out = nullptr, std::vector<int32_t>* h_its_out = nullptr,
+							 std::vector<uint64_t>* tspan_out = nullptr)
 {
 	krylov_result_t kr;
 	std::vector<int32_t> h_its, h_cyc;
 	std::vector<double> h_res;
-	read_host(c, h_its, sc.its, B);
-	read_host(c, h_cyc, sc.cyc, B);
-	read_host(c, h_res, sc.res, B);
+	read_set_outputs(c, sc, B, h_res, h_its, h_cyc, tspan_out);
 	for (int b = 0; b < B; ++b)
 	{
 		kr.resid = std::max(kr.resid, std::isnan(h_res[b]) ? 1e300 : h_res[b]);
@@ -4024,8 +4059,7 @@ krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, 
 	const int B = S.B;
 	set_results_t sr;
 	sr.method.assign(B, 0);
-	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its);
-	read_host(c, sr.tspan, sc.tspan, (size_t)2 * B);
+	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its, &sr.tspan);
 	if (o.precision == 1 && kr.missed > 0)
 	{
 		o.precision = 0;
@@ -5275,6 +5309,8 @@ struct child_ctx_t
 			ESL_HIP(hipEventCreate(&e));
 		c.flags.ensure(64);
 		ESL_HIP(hipHostMalloc((void**)&c.h_flags, 64 * sizeof(int32_t), hipHostMallocDefault));
+		c.h_stage_bytes = stage_bytes;
+		ESL_HIP(hipHostMalloc((void**)&c.h_stage, c.h_stage_bytes, hipHostMallocDefault));
 	}
 	~child_ctx_t()
 	{
@@ -5289,6 +5325,8 @@ struct child_ctx_t
 				(void)hipEventDestroy(e);
 		if (c.h_flags)
 			(void)hipHostFree(c.h_flags);
+		if (c.h_stage)
+			(void)hipHostFree(c.h_stage);
 	}
 	child_ctx_t(const child_ctx_t&) = delete;
 	child_ctx_t& operator=(const child_ctx_t&) = delete;
