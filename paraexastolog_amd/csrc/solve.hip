@@ -3816,50 +3816,59 @@ std::string sci(double v)
 
 // per-set results of the last launch: max / sums over the sets, and the host
 // copies of the per-set residuals, iteration and cycle counts
-// The per-set outputs of a launch (residuals, iterations, cycles and, when
-// asked, the start / end clocks) in one staged copy: the arrays go back to back
-// into the context's pinned buffer and the host waits once, where a copy per
-// array into pageable memory costs a host round trip each.  A batch too large
-// for the buffer is read one array at a time.
-void read_set_outputs(ctx_t& c, const pss_view_t& sc, int B, std::vector<double>& res, std::vector<int32_t>& its,
-					  std::vector<int32_t>& cyc, std::vector<uint64_t>* tspan)
+// The per-set outputs of a launch (residuals, start / end clocks, iterations,
+// cycles) in one staged copy: the arrays go back to back into the context's
+// pinned buffer, enqueued behind the launch so that one wait covers both, where
+// a copy per array into pageable memory costs a host round trip each.  Returns
+// false (nothing enqueued) when the batch does not fit the buffer.
+bool stage_set_outputs(ctx_t& c, const pss_view_t& sc, int B)
 {
 	const size_t nb = (size_t)std::max(B, 0);
-	const size_t need = nb * (8 + (tspan ? 16 : 0) + 4 + 4);
-	if (!c.h_stage || need > c.h_stage_bytes || nb == 0)
+	if (!c.h_stage || nb == 0 || nb * 32 > c.h_stage_bytes)
+		return false;
+	unsigned char* p = c.h_stage;
+	ESL_HIP(hipMemcpyAsync(p, sc.res, 8 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipMemcpyAsync(p + 8 * nb, sc.tspan, 16 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipMemcpyAsync(p + 24 * nb, sc.its, 4 * nb, hipMemcpyDeviceToHost, c.stream));
+	ESL_HIP(hipMemcpyAsync(p + 28 * nb, sc.cyc, 4 * nb, hipMemcpyDeviceToHost, c.stream));
+	return true;
+}
+
+// staged: stage_set_outputs already enqueued the copies for this view and batch
+void read_set_outputs(ctx_t& c, const pss_view_t& sc, int B, std::vector<double>& res, std::vector<int32_t>& its,
+					  std::vector<int32_t>& cyc, std::vector<uint64_t>* tspan, bool staged)
+{
+	const size_t nb = (size_t)std::max(B, 0);
+	if (staged || stage_set_outputs(c, sc, B))
 	{
-		read_host(c, res, sc.res, nb);
-		read_host(c, its, sc.its, nb);
-		read_host(c, cyc, sc.cyc, nb);
+		ESL_HIP(hipStreamSynchronize(c.stream));
+		const unsigned char* p = c.h_stage;
+		const double* h_res = reinterpret_cast<const double*>(p);
+		const uint64_t* h_ts = reinterpret_cast<const uint64_t*>(p + 8 * nb);
+		const int32_t* h_its = reinterpret_cast<const int32_t*>(p + 24 * nb);
+		const int32_t* h_cyc = h_its + nb;
+		res.assign(h_res, h_res + nb);
+		its.assign(h_its, h_its + nb);
+		cyc.assign(h_cyc, h_cyc + nb);
 		if (tspan)
-			read_host(c, *tspan, sc.tspan, 2 * nb);
+			tspan->assign(h_ts, h_ts + 2 * nb);
 		return;
 	}
-	double* h_res = reinterpret_cast<double*>(c.h_stage);
-	uint64_t* h_ts = reinterpret_cast<uint64_t*>(c.h_stage + 8 * nb);
-	int32_t* h_its = reinterpret_cast<int32_t*>(c.h_stage + 8 * nb + (tspan ? 16 * nb : 0));
-	int32_t* h_cyc = h_its + nb;
-	ESL_HIP(hipMemcpyAsync(h_res, sc.res, 8 * nb, hipMemcpyDeviceToHost, c.stream));
+	read_host(c, res, sc.res, nb);
+	read_host(c, its, sc.its, nb);
+	read_host(c, cyc, sc.cyc, nb);
 	if (tspan)
-		ESL_HIP(hipMemcpyAsync(h_ts, sc.tspan, 16 * nb, hipMemcpyDeviceToHost, c.stream));
-	ESL_HIP(hipMemcpyAsync(h_its, sc.its, 4 * nb, hipMemcpyDeviceToHost, c.stream));
-	ESL_HIP(hipMemcpyAsync(h_cyc, sc.cyc, 4 * nb, hipMemcpyDeviceToHost, c.stream));
-	ESL_HIP(hipStreamSynchronize(c.stream));
-	res.assign(h_res, h_res + nb);
-	its.assign(h_its, h_its + nb);
-	cyc.assign(h_cyc, h_cyc + nb);
-	if (tspan)
-		tspan->assign(h_ts, h_ts + 2 * nb);
+		read_host(c, *tspan, sc.tspan, 2 * nb);
 }
 
 krylov_result_t collect_sets(ctx_t& c, const pss_view_t& sc, int B, double tol, bool check,
 							 std::vector<double>* h_res_out = nullptr, std::vector<int32_t>* h_its_out = nullptr,
-							 std::vector<uint64_t>* tspan_out = nullptr)
+							 std::vector<uint64_t>* tspan_out = nullptr, bool staged = false)
 {
 	krylov_result_t kr;
 	std::vector<int32_t> h_its, h_cyc;
 	std::vector<double> h_res;
-	read_set_outputs(c, sc, B, h_res, h_its, h_cyc, tspan_out);
+	read_set_outputs(c, sc, B, h_res, h_its, h_cyc, tspan_out, staged);
 	for (int b = 0; b < B; ++b)
 	{
 		kr.resid = std::max(kr.resid, std::isnan(h_res[b]) ? 1e300 : h_res[b]);
@@ -4053,13 +4062,13 @@ void launch_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, 
 template <typename F>
 krylov_result_t finish_per_set(ctx_t& c, const sys_t& S, const nbr_store_t& nb, int n_seg, const double* rhs,
 							   double* z, esl_solve_opts o, const pss_view_t& sc, F&& fp64_view,
-							   set_results_t* out = nullptr)
+							   set_results_t* out = nullptr, bool staged = false)
 {
 	ESL_HIP(hipStreamSynchronize(c.stream));
 	const int B = S.B;
 	set_results_t sr;
 	sr.method.assign(B, 0);
-	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its, &sr.tspan);
+	krylov_result_t kr = collect_sets(c, sc, B, o.tol, false, &sr.res, &sr.its, &sr.tspan, staged);
 	if (o.precision == 1 && kr.missed > 0)
 	{
 		o.precision = 0;
@@ -5008,13 +5017,14 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 				P.gprof = c.prof.on;
 			}
 			ESL_HIP(hipGraphLaunch(P.gexec, c.stream));
-			ESL_HIP(hipStreamSynchronize(c.stream));
-			if (c.prof.on)
-				c.prof.book(P.gpend);
 		}
 		else
 			plan_enqueue(c, P, B, d_rates, d_final, o, true);
+		// the per-set outputs follow the launch on the stream: one wait for both
+		const bool staged = P.nU > 0 && stage_set_outputs(c, P.rb.ps, B);
 		ESL_HIP(hipStreamSynchronize(c.stream));
+		if (use_graph && c.prof.on)
+			c.prof.book(P.gpend);
 		if (P.nU > 0)
 		{
 			esl_solve_opts of = o;
@@ -5024,7 +5034,7 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 				P.scr.reserve(B, P.nU, m, 0);
 				P.use_own_bufs();
 				return P.rb.ps;
-			}, &sr);
+			}, &sr, staged);
 			// the final vectors are rebuilt when FP64 iterates replaced FP32 ones
 			plan_finish_sets(c, P, B, d_rates, d_final, of, kr, std::move(sr), kr.fp64_fallback > 0, true);
 		}
