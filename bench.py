@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--sets", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=None,
                     help="network seed (default: 0 for n20 and uneven, 11 for n16 and warm, 8 for n22)")

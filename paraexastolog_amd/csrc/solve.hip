@@ -4962,7 +4962,9 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 		o.tol = 1e-13;
 	if (o.max_iters <= 0)
 		o.max_iters = 20000;
-	timer_t tm_(c);
+	// host wall time of the call (the stream is idle at its end); device events
+	// would add their creation and one more host round trip to every solve
+	const auto t_start = std::chrono::steady_clock::now();
 	const bool tuned = o.restart == 0 && P.tune.set;
 	if (tuned)
 		o.restart = P.tune.m;
@@ -5062,7 +5064,8 @@ void plan_run(ctx_t& c, plan_t& P, int B, const double* d_rates, double* d_final
 	rep.fp64_fallback = kr.fp64_fallback;
 	rep.iterations_sum = kr.iters_sum;
 	rep.exact_solves = kr.exact;
-	rep.ms_stage[3] = tm_.lap();
+	ESL_HIP(hipStreamSynchronize(c.stream));
+	rep.ms_stage[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 	rep.ms_total = rep.ms_stage[3];
 }
 
@@ -5578,7 +5581,8 @@ void plan_gather_terminal(ctx_t& c, const void* p, int B, const double* d_final,
 // global memory / device-wide / per-set LDS-resident with the basis in global
 // memory / per-set LDS-resident with the private basis at 3 or 2 workgroups per
 // CU, restart length 16 / 24 / 32 / 64.  Each candidate runs once untimed
-// (graph capture, buffer growth) and three times timed (best of three).  The
+// (graph capture, buffer growth), then two timed blocks of four runs (the
+// better block's mean).  The
 // chosen schedule then runs once more, so d_final holds its results.  Later
 // solves use the choice when they ask for restart 0.
 void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_final, const esl_solve_opts& o_in,
@@ -5621,12 +5625,17 @@ void plan_autotune(ctx_t& c, void* p, int B, const double* d_rates, double* d_fi
 		{
 			esl_solve_report rep{};
 			plan_run(c, P, B, d_rates, d_final, o, rep);
+			// the better of two blocks of four back-to-back runs (mean per run):
+			// single runs vary by about 5 %, and the best single run favours
+			// whichever candidate had one lucky run over the steady-state rate
 			double t = 1e300;
-			for (int r = 0; r < 3; ++r) // the best of three: single runs vary by about 5 %
+			for (int blk = 0; blk < 2; ++blk)
 			{
 				const auto t0 = std::chrono::steady_clock::now();
-				plan_run(c, P, B, d_rates, d_final, o, rep);
-				t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+				for (int r = 0; r < 4; ++r)
+					plan_run(c, P, B, d_rates, d_final, o, rep);
+				t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+									4.0);
 			}
 			if (t < best)
 			{
