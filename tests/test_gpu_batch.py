@@ -65,6 +65,20 @@ def test_large_batches(ctx):
         s = solver(ctx, t, g, rates[k], x0)
         s.solve()
         assert np.abs(xb[k] - s.final_state).sum() <= 1e-10
+    # more sets than the context's pinned staging buffer holds (8,192): the
+    # per-set outputs come back array by array
+    Bl = 9000
+    rl = np.concatenate([rates, rates, rates, rates])[:Bl]
+    drl = DeviceArray.from_host(ctx, rl)
+    dl = DeviceArray(ctx, (Bl, n), np.float64)
+    rep = plan.solve(Bl, drl.ptr, dl.ptr, precision=1, tol=1e-12)
+    assert rep["residual"] <= 1e-12
+    res, its, meth = plan.set_results(Bl)
+    assert len(res) == Bl and res.max() <= 1e-12 and (its > 0).all() and (meth >= 0).all()
+    xl = dl.download()
+    for k in (0, 2499, 2500, 8191, 8192, Bl - 1):
+        assert np.abs(xl[k] - xb[k % B]).sum() <= 1e-10
+    del dl, drl, xl
     Bw = 1100
     dw = DeviceArray(ctx, (Bw, n), np.float64)
     plan.set_schedule(2, 32)
