@@ -113,11 +113,13 @@ def stage_bandwidth(stats, n, m, counts=None):
     return out
 
 
-def traffic_record(kernel, workload, restart=None):
+def traffic_record(kernel, workload, restart=None, name="traffic.json"):
     """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE passes, scaled by the same counters on a 1 GiB copy), when a
-    record for this kernel, workload and restart length exists under profiles/."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    record for this kernel, workload and restart length exists under profiles/
+    (name="issue.json": its VALU instruction count and issue fraction, from the
+    SQ counters)."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             recs = json.load(f)
@@ -575,6 +577,15 @@ def main():
     if tr is not None:
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
         out["roofline"]["traffic_source"] = tr["source"]
+    # the issue roof beside the HBM one: VALU instructions per launch against one
+    # wave instruction per CU per cycle (rocprofv3 SQ counters, kernel alone)
+    ir = traffic_record(dom[0], f"N=20 seed {a.seed} sets {B}", (schedule or {}).get("restart"), "issue.json")
+    if ir is not None:
+        out["issue_roof"] = {"bound": "valu issue", "valu_insts_per_launch": ir["valu_insts_per_launch"],
+                             "frac_alone": ir["valu_issue_frac"], "launch_ms_counted": ir["launch_ms_counted"],
+                             "frac_in_step": (ir["valu_insts_per_launch"] / (256 * 2.4e9 * avg_ms * 1e-3)
+                                              if avg_ms > 0 else None),
+                             "lds_bank_conflict_frac": ir["lds_bank_conflict_frac"], "source": ir["source"]}
     print(json.dumps(out))
     sys.stdout.flush()
     if a.calibrate:
